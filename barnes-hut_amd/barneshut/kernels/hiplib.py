@@ -1,0 +1,292 @@
+"""ctypes binding of libbh_hip.so (the C ABI declared in include/bh_hip.h).
+
+PyTorch is used only for device buffers and the current HIP stream.  There is
+no CPU fallback: if the shared library is missing, or no GPU is present, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                     "csrc")
+LIB_PATH = os.path.join(_CSRC, "libbh_hip.so")
+
+BH_OK, BH_EINVAL, BH_EHIP, BH_EDEVICE, BH_ESTATE = 0, 1, 2, 3, 4
+PREC_FP64, PREC_FP32_KAHAN, PREC_FP32_KAHAN_TERM, PREC_FP32_PLAIN = 0, 1, 2, 3
+PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32_KAHAN, "fp32_kahan": PREC_FP32_KAHAN,
+              "fp32_kahan_term": PREC_FP32_KAHAN_TERM, "fp32_plain": PREC_FP32_PLAIN}
+(EXPORT_COUNTS, EXPORT_OFFSETS, EXPORT_NEPREFIX, EXPORT_PERM, EXPORT_KEYS, EXPORT_COM,
+ EXPORT_BBOX, EXPORT_ITEMS, EXPORT_COST, EXPORT_KEYS_IN) = range(10)
+MLP_NPARAM = 1377
+MLP_NFEAT = 8
+ST_ERR, ST_MAXCOUNT, ST_NITEMS, ST_QUEUE, ST_NCELLS_NE = 0, 1, 2, 3, 4
+
+# every entry point declared in include/bh_hip.h
+EXPORTED = (
+    "bh_last_error", "bh_version", "bh_create", "bh_destroy", "bh_set_systems", "bh_set_tuning",
+    "bh_get_tuning", "bh_set_cost_log", "bh_place_particles", "bh_calculate_box_cumm",
+    "bh_sort_particles", "bh_summarize", "bh_build_items", "bh_evaluate_boxes",
+    "bh_tick_particles", "bh_step", "bh_step_graph", "bh_read_stats", "bh_export",
+    "bh_interactions", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
+    "bh_order_by_cost", "bh_learned_order", "bh_query",
+)
+
+
+class BhError(RuntimeError):
+    """A libbh_hip call returned an error code."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libbh_hip error {code}: {msg}")
+        self.code = code
+
+
+class BhParticles(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("px", "py", "mass", "vx", "vy", "ax", "ay", "pid")]
+
+
+class BhLaw(ctypes.Structure):
+    _fields_ = [("G", ctypes.c_double), ("eps2", ctypes.c_double), ("r2_cut", ctypes.c_double)]
+
+
+class BhTuning(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_int32), ("yspan", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libbh_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (python -c "
+                           f"'import __graft_entry__ as g; g.build()' or make -C {_CSRC})")
+    L = ctypes.CDLL(path)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    P = ctypes.POINTER
+    sig = {
+        "bh_last_error": (ctypes.c_char_p, []),
+        "bh_version": (ctypes.c_int, []),
+        "bh_create": (ctypes.c_int, [ctypes.c_int, P(vp)]),
+        "bh_destroy": (ctypes.c_int, [vp]),
+        "bh_set_systems": (ctypes.c_int, [vp, i32, P(i64), P(i32)]),
+        "bh_set_tuning": (ctypes.c_int, [vp, P(BhTuning)]),
+        "bh_get_tuning": (ctypes.c_int, [vp, P(BhTuning)]),
+        "bh_set_cost_log": (ctypes.c_int, [vp, i32]),
+        "bh_place_particles": (ctypes.c_int, [vp, P(BhParticles), vp]),
+        "bh_calculate_box_cumm": (ctypes.c_int, [vp, vp]),
+        "bh_sort_particles": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), vp]),
+        "bh_summarize": (ctypes.c_int, [vp, P(BhParticles), vp]),
+        "bh_build_items": (ctypes.c_int, [vp, vp]),
+        "bh_evaluate_boxes": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), i32, vp, vp]),
+        "bh_tick_particles": (ctypes.c_int, [vp, P(BhParticles), dbl, vp]),
+        "bh_step": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), P(BhLaw), i32, dbl, i32,
+                                   vp]),
+        "bh_step_graph": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), P(BhLaw), i32, dbl,
+                                         i32, vp]),
+        "bh_read_stats": (ctypes.c_int, [vp, P(i32), vp]),
+        "bh_export": (ctypes.c_int, [vp, i32, vp, i64, vp]),
+        "bh_interactions": (ctypes.c_int, [vp, P(i64), vp]),
+        "bh_item_features": (ctypes.c_int, [vp, vp, vp]),
+        "bh_mlp_forward": (ctypes.c_int, [vp, vp, i64, vp, vp]),
+        "bh_mlp_backward": (ctypes.c_int, [vp, vp, vp, i64, vp, vp, vp]),
+        "bh_mlp_adam": (ctypes.c_int, [vp, vp, vp, vp, i32, ctypes.c_float, vp]),
+        "bh_order_by_cost": (ctypes.c_int, [vp, vp, vp, vp]),
+        "bh_learned_order": (ctypes.c_int, [vp, vp, vp, vp]),
+        "bh_query": (ctypes.c_int, [vp, i32, P(i64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != BH_OK:
+        msg = _lib.bh_last_error().decode() if _lib is not None else "?"
+        raise BhError(code, msg)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def particles_struct(d):
+    """BhParticles from a dict of device tensors px, py, mass, vx, vy, ax, ay, pid."""
+    for k in ("px", "py", "mass", "vx", "vy", "ax", "ay"):
+        t = d[k]
+        if t.dtype != _torch().float64 or not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{k} must be a contiguous float64 device tensor")
+    t = d["pid"]
+    if t.dtype != _torch().int32 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("pid must be a contiguous int32 device tensor")
+    return BhParticles(*[d[k].data_ptr() for k in ("px", "py", "mass", "vx", "vy", "ax", "ay",
+                                                   "pid")])
+
+
+class Context:
+    """Owns one bh_ctx (device scratch for a set of independent systems)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = ctypes.c_void_p()
+        check(self.L.bh_create(int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+        self.n_sys = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.bh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- setup
+    def set_systems(self, n, grid_dim):
+        n = np.ascontiguousarray(np.atleast_1d(n), dtype=np.int64)
+        g = np.ascontiguousarray(np.atleast_1d(grid_dim), dtype=np.int32)
+        if n.shape != g.shape:
+            raise ValueError("n and grid_dim must have the same length")
+        check(self.L.bh_set_systems(self.h, len(n), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                    g.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        self.n_sys = len(n)
+        self.n = n.copy()
+        self.grid_dim = g.copy()
+
+    def set_tuning(self, p=2, yspan=16, waves_per_cu=16):
+        t = BhTuning(int(p), int(yspan), int(waves_per_cu))
+        check(self.L.bh_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = BhTuning()
+        check(self.L.bh_get_tuning(self.h, ctypes.byref(t)))
+        return dict(p=t.p, yspan=t.yspan, waves_per_cu=t.waves_per_cu)
+
+    def set_cost_log(self, on: bool):
+        check(self.L.bh_set_cost_log(self.h, 1 if on else 0))
+
+    def query(self, what: int) -> int:
+        out = ctypes.c_int64()
+        check(self.L.bh_query(self.h, int(what), ctypes.byref(out)))
+        return int(out.value)
+
+    # ---- phases
+    def place(self, parts, stream=None):
+        p = particles_struct(parts)
+        check(self.L.bh_place_particles(self.h, ctypes.byref(p), stream_handle(stream)))
+
+    def cumm(self, stream=None):
+        check(self.L.bh_calculate_box_cumm(self.h, stream_handle(stream)))
+
+    def sort(self, src, dst, stream=None):
+        a, b = particles_struct(src), particles_struct(dst)
+        check(self.L.bh_sort_particles(self.h, ctypes.byref(a), ctypes.byref(b),
+                                       stream_handle(stream)))
+
+    def summarize(self, parts, stream=None):
+        p = particles_struct(parts)
+        check(self.L.bh_summarize(self.h, ctypes.byref(p), stream_handle(stream)))
+
+    def build_items(self, stream=None):
+        check(self.L.bh_build_items(self.h, stream_handle(stream)))
+
+    def evaluate(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", order=None,
+                 stream=None):
+        p = particles_struct(parts)
+        law = BhLaw(float(G), float(eps2), float(r2_cut))
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(self.L.bh_evaluate_boxes(self.h, ctypes.byref(p), ctypes.byref(law), prec,
+                                       ptr(order), stream_handle(stream)))
+
+    def tick(self, parts, dt, stream=None):
+        p = particles_struct(parts)
+        check(self.L.bh_tick_particles(self.h, ctypes.byref(p), float(dt), stream_handle(stream)))
+
+    def step(self, a, b, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", dt=0.0, do_tick=True,
+             graph=False, stream=None):
+        pa, pb = particles_struct(a), particles_struct(b)
+        law = BhLaw(float(G), float(eps2), float(r2_cut))
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        fn = self.L.bh_step_graph if graph else self.L.bh_step
+        check(fn(self.h, ctypes.byref(pa), ctypes.byref(pb), ctypes.byref(law), prec, float(dt),
+                 1 if do_tick else 0, stream_handle(stream)))
+
+    # ---- state
+    def stats(self, stream=None, raise_on_error=True):
+        out = np.zeros(16, np.int32)
+        code = self.L.bh_read_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                    stream_handle(stream))
+        if code == BH_EDEVICE and not raise_on_error:
+            return out
+        check(code)
+        return out
+
+    def export(self, what, count, dtype, stream=None):
+        torch = _torch()
+        out = torch.empty(int(count), dtype=dtype, device=f"cuda:{self.device}")
+        check(self.L.bh_export(self.h, int(what), ptr(out), int(count), stream_handle(stream)))
+        return out
+
+    def export_host(self, what, count, dtype):
+        out = np.empty(int(count), dtype=dtype)
+        check(self.L.bh_export(self.h, int(what), ctypes.c_void_p(out.ctypes.data), int(count),
+                               stream_handle()))
+        return out
+
+    def interactions(self, stream=None):
+        out = np.zeros(2 * self.n_sys + 2, np.int64)
+        check(self.L.bh_interactions(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     stream_handle(stream)))
+        return out
+
+    # ---- cost model
+    def item_features(self, feat, stream=None):
+        check(self.L.bh_item_features(self.h, ptr(feat), stream_handle(stream)))
+
+    def order_by_cost(self, cost, order, stream=None):
+        check(self.L.bh_order_by_cost(self.h, ptr(cost), ptr(order), stream_handle(stream)))
+
+    def learned_order(self, params, order, stream=None):
+        check(self.L.bh_learned_order(self.h, ptr(params), ptr(order), stream_handle(stream)))
+
+
+def mlp_forward(params, x, y, stream=None):
+    L = load()
+    check(L.bh_mlp_forward(ptr(params), ptr(x), int(x.shape[0]), ptr(y), stream_handle(stream)))
+
+
+def mlp_backward(params, x, target, grad, loss=None, stream=None):
+    L = load()
+    check(L.bh_mlp_backward(ptr(params), ptr(x), ptr(target), int(x.shape[0]), ptr(grad),
+                            ptr(loss), stream_handle(stream)))
+
+
+def mlp_adam(params, grad, m, v, t, lr, stream=None):
+    L = load()
+    check(L.bh_mlp_adam(ptr(params), ptr(grad), ptr(m), ptr(v), int(t), float(lr),
+                        stream_handle(stream)))

@@ -1,0 +1,687 @@
+// C ABI of libbh_hip.so: context, scratch ownership, phase entry points and
+// HIP-graph replay.  The declarations and the reference interface each entry
+// point replaces are in include/bh_hip.h.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bh_hip.h"
+#include "bh_common.h"
+#include "bh_kernels.h"
+
+const void* bh_force_kernel(int precision, int pidx, int law);
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(BH_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+#define LAUNCH_CHECK(what)                                                                     \
+    do {                                                                                       \
+        hipError_t e_ = hipGetLastError();                                                     \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(BH_EHIP, std::string("launch ") + what + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct GraphKey {
+    bh_particles a, b;
+    bh_law law;
+    int precision;
+    double dt;
+    int do_tick;
+    bh_tuning tune;
+    bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+struct bh_ctx_s {
+    int device = 0;
+    int num_cu = 256;
+    int S = 0;
+    int64_t N = 0, C = 0, ncols = 0, maxN = 0;
+    std::vector<SysDev> sys_h;
+    SysDev* sys = nullptr;
+    int32_t* sys_of = nullptr;
+    uint64_t* bb = nullptr;
+    int32_t *key = nullptr, *key_sorted = nullptr, *counts = nullptr, *cursor = nullptr;
+    int32_t *offsets = nullptr, *neprefix = nullptr, *ne_list = nullptr;
+    int32_t *idx_a = nullptr, *idx_b = nullptr, *perm = nullptr;
+    int32_t *bsum = nullptr, *bne = nullptr;
+    int32_t* stats = nullptr;
+    double* com_dense = nullptr;
+    Com64* l64 = nullptr;
+    Com32* l32 = nullptr;
+    Item* items = nullptr;
+    int64_t max_items = 0;
+    int32_t* col_items = nullptr;
+    int32_t* col_off = nullptr;
+    long long* cost_log = nullptr;
+    int cost_log_on = 0;
+    unsigned long long* inter = nullptr;
+    int32_t* cost_scratch = nullptr;
+    float* feat = nullptr;
+    float* pred = nullptr;
+    bh_tuning tune = {2, 16, 16};
+    hipStream_t cap_stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    GraphKey gkey[2];
+    int gnext = 0;
+    std::vector<void*> allocs;
+};
+typedef bh_ctx_s Ctx;
+
+static void free_all(Ctx* c)
+{
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    for (auto& g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+template <typename T>
+static int dalloc(Ctx* c, T** p, int64_t count)
+{
+    size_t bytes = (size_t)std::max<int64_t>(count, 1) * sizeof(T);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(BH_EHIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return BH_OK;
+}
+
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+static inline unsigned grid_for(int64_t n, int bs, int64_t cap = 1 << 20)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, bs), cap));
+}
+
+extern "C" {
+
+const char* bh_last_error(void) { return g_err.c_str(); }
+int bh_version(void) { return 1; }
+
+int bh_create(int device, void** ctx_out)
+{
+    if (!ctx_out) return fail(BH_EINVAL, "ctx_out is NULL");
+    int ndev = 0;
+    HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(BH_EINVAL, "device " + std::to_string(device) + " out of range");
+    HIP_TRY(hipSetDevice(device));
+    Ctx* c = new Ctx();
+    c->device = device;
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+        cu > 0)
+        c->num_cu = cu;
+    if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming) != hipSuccess) {
+        delete c;
+        return fail(BH_EHIP, "stream/event creation failed");
+    }
+    *ctx_out = c;
+    return BH_OK;
+}
+
+int bh_destroy(void* ctx)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return BH_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    free_all(c);
+    if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
+    if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+    if (c->ev_out) (void)hipEventDestroy(c->ev_out);
+    delete c;
+    return BH_OK;
+}
+
+int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* grid_dim)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return fail(BH_EINVAL, "ctx is NULL");
+    if (nsys <= 0 || !n || !grid_dim) return fail(BH_EINVAL, "need nsys >= 1 and host arrays");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipDeviceSynchronize());
+    free_all(c);
+    c->sys_h.assign(nsys, SysDev{});
+    int64_t N = 0, C = 0, cols = 0, maxN = 0;
+    for (int s = 0; s < nsys; ++s) {
+        if (n[s] < 1) return fail(BH_EINVAL, "system " + std::to_string(s) + " has n < 1");
+        if (grid_dim[s] < 1 || grid_dim[s] > 46340)
+            return fail(BH_EINVAL, "system " + std::to_string(s) + " grid_dim out of [1, 46340]");
+        SysDev& d = c->sys_h[s];
+        d.p_off = N;
+        d.n = n[s];
+        d.c_off = C;
+        d.col_off = cols;
+        d.G = grid_dim[s];
+        N += n[s];
+        C += (int64_t)grid_dim[s] * grid_dim[s];
+        cols += grid_dim[s];
+        maxN = std::max(maxN, n[s]);
+    }
+    if (N >= (1ll << 31) - 1 || C >= (1ll << 31) - 1)
+        return fail(BH_EINVAL, "total particles and cells must each be < 2^31");
+    c->S = nsys;
+    c->N = N;
+    c->C = C;
+    c->ncols = cols;
+    c->maxN = maxN;
+    c->max_items = C + cols + 2 * cdiv(N, 64) + 64;
+    int64_t nb = std::max(cdiv(C, SCAN_TILE), cdiv(cols, SCAN_TILE)) + 1;
+    int r = BH_OK;
+#define A(p, cnt) \
+    if ((r = dalloc(c, &c->p, (cnt))) != BH_OK) return r
+    A(sys, nsys);
+    A(sys_of, N);
+    A(bb, 4 * (int64_t)nsys);
+    A(key, N);
+    A(key_sorted, N);
+    A(counts, C);
+    A(cursor, C);
+    A(offsets, C + 1);
+    A(neprefix, C + 1);
+    A(ne_list, C);
+    A(idx_a, N);
+    A(idx_b, N);
+    A(perm, N);
+    A(bsum, nb);
+    A(bne, nb);
+    A(stats, ST_COUNT);
+    A(com_dense, 3 * C);
+    A(l64, C);
+    A(l32, C);
+    A(items, c->max_items);
+    A(col_items, cols);
+    A(col_off, cols + 1);
+    A(cost_log, c->max_items);
+    A(inter, 2 * (int64_t)nsys + 2);
+    A(cost_scratch, 16 + 3 * 4096 + 1);
+    A(feat, BH_MLP_NFEAT * c->max_items);
+    A(pred, c->max_items);
+#undef A
+    HIP_TRY(hipMemcpy(c->sys, c->sys_h.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice));
+    std::vector<int32_t> so(N);
+    for (int s = 0; s < nsys; ++s)
+        std::fill(so.begin() + c->sys_h[s].p_off, so.begin() + c->sys_h[s].p_off + c->sys_h[s].n, s);
+    HIP_TRY(hipMemcpy(c->sys_of, so.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(c->stats, 0, sizeof(int32_t) * ST_COUNT));
+    HIP_TRY(hipMemset(c->offsets, 0, sizeof(int32_t) * (C + 1)));
+    HIP_TRY(hipMemset(c->neprefix, 0, sizeof(int32_t) * (C + 1)));
+    HIP_TRY(hipDeviceSynchronize());
+    return BH_OK;
+}
+
+int bh_set_tuning(void* ctx, const bh_tuning* t)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !t) return fail(BH_EINVAL, "NULL argument");
+    if (!(t->p == 1 || t->p == 2 || t->p == 4)) return fail(BH_EINVAL, "tuning.p must be 1, 2 or 4");
+    if (t->yspan < 1 || t->yspan > 1 << 20) return fail(BH_EINVAL, "tuning.yspan must be >= 1");
+    if (t->waves_per_cu < 1 || t->waves_per_cu > 64)
+        return fail(BH_EINVAL, "tuning.waves_per_cu must be in [1, 64]");
+    c->tune = *t;
+    return BH_OK;
+}
+
+int bh_get_tuning(void* ctx, bh_tuning* t)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !t) return fail(BH_EINVAL, "NULL argument");
+    *t = c->tune;
+    return BH_OK;
+}
+
+int bh_set_cost_log(void* ctx, int32_t enable)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return fail(BH_EINVAL, "ctx is NULL");
+    c->cost_log_on = enable ? 1 : 0;
+    return BH_OK;
+}
+
+static int check_parts(const bh_particles* p, const char* name)
+{
+    if (!p) return fail(BH_EINVAL, std::string(name) + " is NULL");
+    if (!p->px || !p->py || !p->mass || !p->vx || !p->vy || !p->ax || !p->ay || !p->pid)
+        return fail(BH_EINVAL, std::string(name) + " has a NULL field");
+    return BH_OK;
+}
+
+#define CTX_READY(c)                                                        \
+    do {                                                                    \
+        if (!(c)) return fail(BH_EINVAL, "ctx is NULL");                    \
+        if ((c)->S == 0) return fail(BH_ESTATE, "bh_set_systems not called"); \
+    } while (0)
+
+int bh_place_particles(void* ctx, const bh_particles* p, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(p, "particles");
+    if (r) return r;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t N = c->N;
+    unsigned nchunk = (unsigned)cdiv(N, PLACE_CHUNK);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats, (int64_t)ST_COUNT, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->C, 256, 4096)), dim3(256), 0, st, c->counts,
+                       c->C, 0);
+    hipLaunchKernelGGL(k_bbox_init, dim3(cdiv(c->S, 256)), dim3(256), 0, st, c->bb, c->S);
+    hipLaunchKernelGGL(k_bbox, dim3(nchunk), dim3(256), 0, st, (const double*)p->px,
+                       (const double*)p->py, (const int32_t*)c->sys_of, N, c->bb, c->stats);
+    hipLaunchKernelGGL(k_bbox_finalize, dim3(cdiv(c->S, 256)), dim3(256), 0, st, c->sys,
+                       (const uint64_t*)c->bb, c->S, c->stats);
+    hipLaunchKernelGGL(k_keys_hist, dim3(nchunk), dim3(256), 0, st, (const double*)p->px,
+                       (const double*)p->py, (const int32_t*)c->sys_of, (const SysDev*)c->sys, N,
+                       c->key, c->counts);
+    LAUNCH_CHECK("place");
+    return BH_OK;
+}
+
+static void run_scan(Ctx* c, bool cnt, const int32_t* in, int64_t n, int32_t* out_v,
+                     int32_t* out_f, int total_slot, hipStream_t st)
+{
+    unsigned nb = (unsigned)cdiv(n, SCAN_TILE);
+    if (cnt) {
+        hipLaunchKernelGGL(k_scan_reduce<true>, dim3(nb), dim3(256), 0, st, in, n, c->bsum, c->bne,
+                           c->stats);
+        hipLaunchKernelGGL(k_scan_blocks<true>, dim3(1), dim3(1024), 0, st, c->bsum, c->bne,
+                           (int64_t)nb, c->stats, total_slot);
+        hipLaunchKernelGGL(k_scan_final<true>, dim3(nb), dim3(256), 0, st, in, n,
+                           (const int32_t*)c->bsum, (const int32_t*)c->bne, out_v, out_f);
+    } else {
+        hipLaunchKernelGGL(k_scan_reduce<false>, dim3(nb), dim3(256), 0, st, in, n, c->bsum,
+                           c->bne, c->stats);
+        hipLaunchKernelGGL(k_scan_blocks<false>, dim3(1), dim3(1024), 0, st, c->bsum, c->bne,
+                           (int64_t)nb, c->stats, total_slot);
+        hipLaunchKernelGGL(k_scan_final<false>, dim3(nb), dim3(256), 0, st, in, n,
+                           (const int32_t*)c->bsum, (const int32_t*)c->bne, out_v, out_f);
+    }
+}
+
+int bh_calculate_box_cumm(void* ctx, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    hipStream_t st = (hipStream_t)stream;
+    run_scan(c, true, c->counts, c->C, c->offsets, c->neprefix, -1, st);
+    LAUNCH_CHECK("scan");
+    return BH_OK;
+}
+
+static SortFields fields_of(const bh_particles* p)
+{
+    SortFields f;
+    f.px = p->px;
+    f.py = p->py;
+    f.m = p->mass;
+    f.vx = p->vx;
+    f.vy = p->vy;
+    f.ax = p->ax;
+    f.ay = p->ay;
+    f.pid = p->pid;
+    return f;
+}
+
+static BhParticles dev_of(const bh_particles* p)
+{
+    BhParticles d;
+    d.px = p->px;
+    d.py = p->py;
+    d.m = p->mass;
+    d.vx = p->vx;
+    d.vy = p->vy;
+    d.ax = p->ax;
+    d.ay = p->ay;
+    d.pid = p->pid;
+    return d;
+}
+
+int bh_sort_particles(void* ctx, const bh_particles* src, const bh_particles* dst, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(src, "src");
+    if (r) return r;
+    if ((r = check_parts(dst, "dst"))) return r;
+    if (src->px == dst->px) return fail(BH_EINVAL, "src and dst must be different buffers");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t N = c->N;
+    SortFields F = fields_of(src);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->C, 256, 4096)), dim3(256), 0, st, c->cursor,
+                       c->C, 0);
+    hipLaunchKernelGGL(k_scatter, dim3(cdiv(N, PLACE_CHUNK)), dim3(256), 0, st,
+                       (const int32_t*)c->key, N, (const int32_t*)c->offsets, c->cursor, c->idx_a,
+                       c->key_sorted);
+    hipLaunchKernelGGL(k_sort_units, dim3(cdiv(N, SORT_W)), dim3(256), 0, st, F, N,
+                       (const int32_t*)c->idx_a, (const int32_t*)c->key_sorted,
+                       (const int32_t*)c->offsets, c->idx_b);
+    // merge passes for cells longer than SORT_W: b -> a -> b -> ...
+    int maxpass = 0;
+    while (((int64_t)SORT_W << maxpass) < c->maxN) ++maxpass;
+    unsigned mgrid = grid_for(N, 256, (int64_t)c->num_cu * 8);
+    for (int L = 0; L < maxpass; ++L) {
+        const int32_t* in = (L & 1) ? c->idx_a : c->idx_b;
+        int32_t* out = (L & 1) ? c->idx_b : c->idx_a;
+        hipLaunchKernelGGL(k_sort_merge, dim3(mgrid), dim3(256), 0, st, F, N, in,
+                           (const int32_t*)c->key_sorted, (const int32_t*)c->offsets, out, L,
+                           (const int32_t*)c->stats);
+    }
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(N, 256, (int64_t)c->num_cu * 16)), dim3(256), 0, st,
+                       F, dev_of(dst), N, (const int32_t*)c->idx_a, (const int32_t*)c->idx_b,
+                       (const int32_t*)c->key_sorted, (const int32_t*)c->offsets,
+                       (const int32_t*)c->stats, c->perm);
+    LAUNCH_CHECK("sort");
+    return BH_OK;
+}
+
+int bh_summarize(void* ctx, const bh_particles* p, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(p, "particles");
+    if (r) return r;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_ne_list, dim3(grid_for(c->C, 256, 8192)), dim3(256), 0, st,
+                       (const int32_t*)c->counts, (const int32_t*)c->neprefix, c->C, c->ne_list);
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(3 * c->C, 256, 8192)), dim3(256), 0, st,
+                       c->com_dense, 3 * c->C, 0.0);
+    // one wave per non-empty cell; the grid is sized for the worst case
+    int64_t waves = std::min<int64_t>(c->C, (int64_t)c->num_cu * 64);
+    hipLaunchKernelGGL(k_com, dim3(grid_for(waves * 64, 256)), dim3(256), 0, st,
+                       (const SysDev*)c->sys, c->S, (const int32_t*)c->ne_list,
+                       (const int32_t*)c->stats, (const int32_t*)c->offsets,
+                       (const double*)p->px, (const double*)p->py, (const double*)p->mass,
+                       c->com_dense, c->l64, c->l32);
+    LAUNCH_CHECK("summarize");
+    return BH_OK;
+}
+
+int bh_build_items(void* ctx, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    hipStream_t st = (hipStream_t)stream;
+    const int pitem = 64 * c->tune.p;
+    unsigned g = (unsigned)cdiv(c->ncols, 64);
+    hipLaunchKernelGGL(k_items<false>, dim3(g), dim3(64), 0, st, (const SysDev*)c->sys, c->S,
+                       c->ncols, (const int32_t*)c->counts, (const int32_t*)c->offsets, pitem,
+                       c->tune.yspan, c->col_items, c->items, c->max_items, c->stats);
+    run_scan(c, false, c->col_items, c->ncols, c->col_off, nullptr, ST_NITEMS, st);
+    hipLaunchKernelGGL(k_clamp_items, dim3(1), dim3(64), 0, st, c->stats, c->max_items);
+    hipLaunchKernelGGL(k_items<true>, dim3(g), dim3(64), 0, st, (const SysDev*)c->sys, c->S,
+                       c->ncols, (const int32_t*)c->counts, (const int32_t*)c->offsets, pitem,
+                       c->tune.yspan, c->col_off, c->items, c->max_items, c->stats);
+    LAUNCH_CHECK("items");
+    return BH_OK;
+}
+
+int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
+                      const int32_t* order, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(p, "particles");
+    if (r) return r;
+    if (!law) return fail(BH_EINVAL, "law is NULL");
+    if (precision < 0 || precision > 3) return fail(BH_EINVAL, "precision must be in [0, 3]");
+    if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
+        return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    if (!order) {
+        if ((r = bh_build_items(ctx, stream))) return r;
+    }
+    const int pidx = c->tune.p == 1 ? 0 : (c->tune.p == 2 ? 1 : 2);
+    const int lawi = law->r2_cut > 0.0 ? 1 : 0;
+    const void* fn = bh_force_kernel(precision, pidx, lawi);
+    if (!fn) return fail(BH_EINVAL, "no force kernel for this configuration");
+    ForceArgs fa;
+    fa.items = c->items;
+    fa.order = order;
+    fa.stats = c->stats;
+    fa.sys = c->sys;
+    fa.offsets = c->offsets;
+    fa.neprefix = c->neprefix;
+    fa.key_sorted = c->key_sorted;
+    fa.x = p->px;
+    fa.y = p->py;
+    fa.m = p->mass;
+    fa.l64 = c->l64;
+    fa.l32 = c->l32;
+    fa.ax = p->ax;
+    fa.ay = p->ay;
+    fa.Gc = law->G;
+    fa.eps2 = law->eps2;
+    fa.r2cut = law->r2_cut;
+    fa.cost_log = c->cost_log_on ? c->cost_log : nullptr;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
+    void* args[] = {&fa};
+    unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
+    LAUNCH_CHECK("evaluate");
+    return BH_OK;
+}
+
+int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(p, "particles");
+    if (r) return r;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tick, dim3(grid_for(c->N, 256, (int64_t)c->num_cu * 16)), dim3(256), 0, st,
+                       p->px, p->py, p->vx, p->vy, p->ax, p->ay, c->N, dt);
+    LAUNCH_CHECK("tick");
+    return BH_OK;
+}
+
+int bh_step(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+            int32_t precision, double dt, int32_t do_tick, void* stream)
+{
+    int r;
+    if ((r = bh_place_particles(ctx, a, stream))) return r;
+    if ((r = bh_calculate_box_cumm(ctx, stream))) return r;
+    if ((r = bh_sort_particles(ctx, a, b, stream))) return r;
+    if ((r = bh_summarize(ctx, b, stream))) return r;
+    if ((r = bh_evaluate_boxes(ctx, b, law, precision, nullptr, stream))) return r;
+    if (do_tick && (r = bh_tick_particles(ctx, b, dt, stream))) return r;
+    return BH_OK;
+}
+
+int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+                  int32_t precision, double dt, int32_t do_tick, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(a, "a");
+    if (r) return r;
+    if ((r = check_parts(b, "b"))) return r;
+    if (!law) return fail(BH_EINVAL, "law is NULL");
+    GraphKey k;
+    std::memset(&k, 0, sizeof(k));
+    k.a = *a;
+    k.b = *b;
+    k.law = *law;
+    k.precision = precision;
+    k.dt = dt;
+    k.do_tick = do_tick;
+    k.tune = c->tune;
+    int slot = -1;
+    for (int i = 0; i < 2; ++i)
+        if (c->gexec[i] && c->gkey[i] == k) slot = i;
+    if (slot < 0) {
+        slot = c->gnext;
+        c->gnext ^= 1;
+        if (c->gexec[slot]) {
+            HIP_TRY(hipGraphExecDestroy(c->gexec[slot]));
+            c->gexec[slot] = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rr = bh_step(ctx, a, b, law, precision, dt, do_tick, c->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
+        if (rr) {
+            if (g) (void)hipGraphDestroy(g);
+            return rr;
+        }
+        if (e != hipSuccess) return fail(BH_EHIP, std::string("capture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&c->gexec[slot], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(BH_EHIP, std::string("instantiate: ") + hipGetErrorString(e));
+        c->gkey[slot] = k;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipEventRecord(c->ev_in, st));
+    HIP_TRY(hipStreamWaitEvent(c->cap_stream, c->ev_in, 0));
+    HIP_TRY(hipGraphLaunch(c->gexec[slot], c->cap_stream));
+    HIP_TRY(hipEventRecord(c->ev_out, c->cap_stream));
+    HIP_TRY(hipStreamWaitEvent(st, c->ev_out, 0));
+    return BH_OK;
+}
+
+int bh_read_stats(void* ctx, int32_t* out16, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!out16) return fail(BH_EINVAL, "out16 is NULL");
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    HIP_TRY(hipMemcpy(out16, c->stats, sizeof(int32_t) * ST_COUNT, hipMemcpyDeviceToHost));
+    if (out16[ST_ERR]) {
+        std::string m = "device error bits " + std::to_string(out16[ST_ERR]) + ":";
+        if (out16[ST_ERR] & BH_ERR_NONFINITE) m += " non-finite particle coordinate;";
+        if (out16[ST_ERR] & BH_ERR_DEGENERATE) m += " zero-extent bounding box;";
+        if (out16[ST_ERR] & BH_ERR_ITEMS) m += " work-item capacity exceeded;";
+        return fail(BH_EDEVICE, m);
+    }
+    return BH_OK;
+}
+
+int bh_export(void* ctx, int32_t what, void* dst, int64_t count, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!dst || count < 0) return fail(BH_EINVAL, "bad dst/count");
+    const void* src = nullptr;
+    int64_t cap = 0;
+    size_t esz = 4;
+    std::vector<double> bbox;
+    switch (what) {
+    case BH_EXPORT_COUNTS: src = c->counts; cap = c->C; break;
+    case BH_EXPORT_OFFSETS: src = c->offsets; cap = c->C + 1; break;
+    case BH_EXPORT_NEPREFIX: src = c->neprefix; cap = c->C + 1; break;
+    case BH_EXPORT_PERM: src = c->perm; cap = c->N; break;
+    case BH_EXPORT_KEYS: src = c->key_sorted; cap = c->N; break;
+    case BH_EXPORT_KEYS_IN: src = c->key; cap = c->N; break;
+    case BH_EXPORT_COM: src = c->com_dense; cap = 3 * c->C; esz = 8; break;
+    case BH_EXPORT_ITEMS: src = c->items; cap = 8 * c->max_items; break;
+    case BH_EXPORT_COST: src = c->cost_log; cap = c->max_items; esz = 8; break;
+    case BH_EXPORT_BBOX: {
+        if (count > 5 * c->S) return fail(BH_EINVAL, "count exceeds 5*S");
+        HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+        std::vector<SysDev> h(c->S);
+        HIP_TRY(hipMemcpy(h.data(), c->sys, sizeof(SysDev) * c->S, hipMemcpyDeviceToHost));
+        bbox.resize(5 * c->S);
+        for (int s = 0; s < c->S; ++s) {
+            bbox[5 * s + 0] = h[s].min_x;
+            bbox[5 * s + 1] = h[s].min_y;
+            bbox[5 * s + 2] = h[s].max_x;
+            bbox[5 * s + 3] = h[s].max_y;
+            bbox[5 * s + 4] = h[s].step;
+        }
+        HIP_TRY(hipMemcpy(dst, bbox.data(), sizeof(double) * count, hipMemcpyDefault));
+        return BH_OK;
+    }
+    default: return fail(BH_EINVAL, "unknown export " + std::to_string(what));
+    }
+    if (count > cap) return fail(BH_EINVAL, "count " + std::to_string(count) + " exceeds " +
+                                                std::to_string(cap));
+    HIP_TRY(hipMemcpyAsync(dst, src, esz * (size_t)count, hipMemcpyDefault, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return BH_OK;
+}
+
+int bh_query(void* ctx, int32_t what, int64_t* out)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c || !out) return fail(BH_EINVAL, "NULL argument");
+    switch (what) {
+    case 0: *out = c->N; break;
+    case 1: *out = c->C; break;
+    case 2: *out = c->S; break;
+    case 3: *out = c->ncols; break;
+    case 4: *out = c->max_items; break;
+    case 5: *out = c->num_cu; break;
+    default: return fail(BH_EINVAL, "unknown query " + std::to_string(what));
+    }
+    return BH_OK;
+}
+
+int bh_interactions(void* ctx, int64_t* out, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!out) return fail(BH_EINVAL, "out is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n2 = 2 * (int64_t)c->S + 2;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, (int32_t*)c->inter, 2 * n2, 0);
+    hipLaunchKernelGGL(k_interactions, dim3(grid_for(c->C, 256, 8192)), dim3(256), 0, st,
+                       (const SysDev*)c->sys, c->S, (const int32_t*)c->counts,
+                       (const int32_t*)c->neprefix, c->C, c->inter);
+    LAUNCH_CHECK("interactions");
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<unsigned long long> h(n2);
+    HIP_TRY(hipMemcpy(h.data(), c->inter, sizeof(unsigned long long) * n2, hipMemcpyDeviceToHost));
+    int64_t ntot = 0;
+    for (int s = 0; s < c->S; ++s) {
+        out[2 * s] = (int64_t)h[2 * s] - c->sys_h[s].n;
+        out[2 * s + 1] = (int64_t)h[2 * s + 1];
+        ntot += c->sys_h[s].n;
+    }
+    out[2 * c->S] = (int64_t)h[2 * c->S] - ntot;
+    out[2 * c->S + 1] = (int64_t)h[2 * c->S + 1];
+    return BH_OK;
+}
+
+}  // extern "C"
+
+// Accessors for the cost-model translation unit.
+Item* bh_ctx_items(void* ctx) { return ((Ctx*)ctx)->items; }
+int32_t* bh_ctx_stats(void* ctx) { return ((Ctx*)ctx)->stats; }
+int64_t bh_ctx_max_items(void* ctx) { return ((Ctx*)ctx)->max_items; }
+int bh_ctx_num_cu(void* ctx) { return ((Ctx*)ctx)->num_cu; }
+int bh_ctx_p(void* ctx) { return ((Ctx*)ctx)->tune.p; }
+const SysDev* bh_ctx_sys(void* ctx) { return ((Ctx*)ctx)->sys; }
+int bh_ctx_S(void* ctx) { return ((Ctx*)ctx)->S; }
+const int32_t* bh_ctx_offsets(void* ctx) { return ((Ctx*)ctx)->offsets; }
+const int32_t* bh_ctx_neprefix(void* ctx) { return ((Ctx*)ctx)->neprefix; }
+const Com64* bh_ctx_l64(void* ctx) { return ((Ctx*)ctx)->l64; }
+int32_t* bh_ctx_scratch_i32(void* ctx, int which)
+{
+    Ctx* c = (Ctx*)ctx;
+    return which == 0 ? c->bsum : (which == 1 ? c->bne : c->cursor);
+}
+int32_t* bh_ctx_cost_scratch(void* ctx) { return ((Ctx*)ctx)->cost_scratch; }
+float* bh_ctx_feat(void* ctx) { return ((Ctx*)ctx)->feat; }
+float* bh_ctx_pred(void* ctx) { return ((Ctx*)ctx)->pred; }
+int bh_ctx_fail(int code, const char* msg) { return fail(code, msg); }

@@ -1,0 +1,560 @@
+// Force evaluation: work items + persistent force kernels (FP64 and FP32).
+//
+// Replaces g_evaluate_boxes (python-bh/barneshut/kernels/grid_decomposition/
+// gpu/grid.py:116-156, with d_is_neighbor :187-192 and the d_*_grav device
+// functions :211-268).  For a particle i in cell (cx, cy):
+//   near: every particle j != i of the 3x3 cell block around (cx, cy),
+//   far:  the COM of every NON-EMPTY cell outside that block (the empty-cell
+//         skip of seq/evaluation.py:86-95),
+//   a_i += Gc * sum m_j (x_j - x_i) w(r2),  w(s) = 1/(s + eps2)  [LAW 0]
+//                                           w(s) = s > r2cut ? 1/(s + eps2) : 0  [LAW 1]
+// and the result is added to the existing acceleration, as the reference
+// accumulates in place (gpu/grid.py:227-230).
+//
+// Work decomposition: a work item is a run of <= 64*P sorted particles of one
+// column cx spanning rows [y0, y1] (y1 - y0 < yspan).  One wave (a 64-thread
+// workgroup) takes items from a queue (atomic cursor; optional order array),
+// stages the item's neighbour particles and then the system's compact COM
+// list through LDS in 64-entry tiles, and every lane evaluates its P
+// particles against each staged entry (LDS broadcast reads).  The COM
+// entries that fall in some lane's 3x3 block lie in three compact ranges
+// (columns cx-1..cx+1, rows y0-1..y1+1); only tiles meeting those ranges
+// take the per-lane masked path.
+#include "bh_common.h"
+#include "bh_kernels.h"
+
+// ---------------------------------------------------------------- items
+// One thread per grid column (all systems).  Greedy packing of the column's
+// non-empty cells, in row order, into items of <= pitem particles and
+// < yspan rows; a cell with more than pitem particles is cut into pitem
+// chunks.  Pass 1 (WRITE = false) counts items per column; pass 2 writes
+// them at the scanned column offsets.
+template <bool WRITE>
+__global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
+                        const int32_t* __restrict__ counts, const int32_t* __restrict__ offsets,
+                        int pitem, int yspan, int32_t* __restrict__ col_items,
+                        Item* __restrict__ items, int64_t max_items, int32_t* stats)
+{
+    int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncols) return;
+    const int s = sys_of_col(sys, S, col);
+    const SysDev& d = sys[s];
+    const int G = d.G;
+    const int x = (int)(col - d.col_off);
+    const int64_t cb = d.c_off + (int64_t)x * G;
+    int64_t out = WRITE ? col_items[col] : 0;
+    int n_out = 0;
+    bool open = false;
+    int p0 = 0, np = 0, y0 = 0, y1 = 0;
+    auto emit = [&](int ip0, int inp, int iy0, int iy1) {
+        if (WRITE) {
+            int64_t k = out + n_out;
+            if (k < max_items) {
+                Item it;
+                it.p0 = ip0;
+                it.np = inp;
+                it.sys = s;
+                it.cx = x;
+                it.y0 = iy0;
+                it.y1 = iy1;
+                it.pad0 = 0;
+                it.pad1 = 0;
+                items[k] = it;
+            } else {
+                atomicOr(&stats[ST_ERR], BH_ERR_ITEMS);
+            }
+        }
+        ++n_out;
+    };
+    for (int yy = 0; yy < G; ++yy) {
+        int nc = counts[cb + yy];
+        if (nc == 0) continue;
+        int pc = offsets[cb + yy];
+        if (open && np + nc <= pitem && yy - y0 < yspan) {
+            np += nc;
+            y1 = yy;
+            continue;
+        }
+        if (open) {
+            emit(p0, np, y0, y1);
+            open = false;
+        }
+        while (nc > pitem) {
+            emit(pc, pitem, yy, yy);
+            pc += pitem;
+            nc -= pitem;
+        }
+        if (nc > 0) {
+            open = true;
+            p0 = pc;
+            np = nc;
+            y0 = yy;
+            y1 = yy;
+        }
+    }
+    if (open) emit(p0, np, y0, y1);
+    if (!WRITE) col_items[col] = n_out;
+}
+
+template __global__ void k_items<false>(const SysDev*, int, int64_t, const int32_t*, const int32_t*,
+                                        int, int, int32_t*, Item*, int64_t, int32_t*);
+template __global__ void k_items<true>(const SysDev*, int, int64_t, const int32_t*, const int32_t*,
+                                       int, int, int32_t*, Item*, int64_t, int32_t*);
+
+// Keeps the item count within the allocated capacity (flags the overflow).
+__global__ void k_clamp_items(int32_t* stats, int64_t max_items)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0 && stats[ST_NITEMS] > max_items) {
+        stats[ST_NITEMS] = (int32_t)max_items;
+        stats[ST_ERR] |= BH_ERR_ITEMS;
+    }
+}
+
+// ---------------------------------------------------------------- common pieces
+struct NearE {
+    double x, y, m;
+    int32_t row, idx;
+};
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v)
+{
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__device__ __forceinline__ int next_item(int32_t* stats, int lane)
+{
+    int q = 0;
+    if (lane == 0) q = atomicAdd(&stats[ST_QUEUE], 1);
+    q = __shfl(q, 0, BH_WAVE);
+    return uniform_int(q);
+}
+
+__device__ __forceinline__ bool meets(int b0, int b1, int lo, int hi)
+{
+    return lo < hi && b0 < hi && b1 > lo;
+}
+
+// One far-field weight M * w(r2): the same expression on the fast and the
+// masked path, so results do not depend on how particles form items.
+template <int LAW>
+__device__ __forceinline__ double far_w64(double wx, double wy, double M, double eps2, double r2cut)
+{
+    if (LAW == 0) return M / fma(wx, wx, fma(wy, wy, eps2));
+    const double s2 = fma(wx, wx, wy * wy);
+    return (s2 > r2cut) ? M / (s2 + eps2) : 0.0;
+}
+
+template <int LAW>
+__device__ __forceinline__ float far_t32(float wx, float wy, float M, float eps2c, float r2c)
+{
+    if (LAW == 0) return M * __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, eps2c)));
+    const float s2 = fmaf(wx, wx, wy * wy);
+    return (s2 > r2c) ? M * __builtin_amdgcn_rcpf(s2 + eps2c) : 0.0f;
+}
+
+// ---------------------------------------------------------------- FP64
+template <int P, int LAW>
+__global__ __launch_bounds__(64) void k_force64(ForceArgs A)
+{
+    __shared__ Com64 ftile[BH_WAVE];
+    __shared__ NearE ntile[BH_WAVE];
+    const int lane = threadIdx.x;
+    const int nitems = A.stats[ST_NITEMS];
+    const double eps2 = A.eps2, r2cut = A.r2cut;
+    for (;;) {
+        const int q = next_item(A.stats, lane);
+        if (q >= nitems) break;
+        const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
+        const int itx = A.order ? uniform_int(A.order[q]) : q;
+        if ((unsigned)itx >= (unsigned)nitems) continue;   // malformed order entry
+        const Item it = A.items[itx];
+        const SysDev& d = A.sys[it.sys];
+        const int G = d.G;
+        const int64_t coff = d.c_off;
+        const int cx = it.cx;
+        const int ylo = max(it.y0 - 1, 0), yhi = min(it.y1 + 1, G - 1);
+
+        double xi[P], yi[P], sx[P], sy[P];
+        int cy[P], gi[P];
+        bool act[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            int slot = lane + BH_WAVE * k;
+            act[k] = slot < it.np;
+            gi[k] = it.p0 + slot;
+            xi[k] = 0.0;
+            yi[k] = 0.0;
+            cy[k] = -100000;
+            sx[k] = 0.0;
+            sy[k] = 0.0;
+            if (act[k]) {
+                xi[k] = A.x[gi[k]];
+                yi[k] = A.y[gi[k]];
+                cy[k] = (int)(A.key_sorted[gi[k]] - coff) - cx * G;
+            }
+        }
+
+        // near field: particles of columns cx-1..cx+1, rows ylo..yhi
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xc = cx + dx;
+            if (xc < 0 || xc >= G) continue;
+            const int64_t cl = coff + (int64_t)xc * G;
+            const int pa = A.offsets[cl + ylo], pb = A.offsets[cl + yhi + 1];
+            for (int base = pa; base < pb; base += BH_WAVE) {
+                const int j = base + lane;
+                NearE e;
+                if (j < pb) {
+                    e.x = A.x[j];
+                    e.y = A.y[j];
+                    e.m = A.m[j];
+                    e.row = (int)(A.key_sorted[j] - cl);
+                    e.idx = j;
+                } else {
+                    e.x = 0.0;
+                    e.y = 0.0;
+                    e.m = 0.0;
+                    e.row = -100000;
+                    e.idx = -1;
+                }
+                ntile[lane] = e;
+                __syncthreads();
+                const int cnt = min(BH_WAVE, pb - base);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const NearE f = ntile[jj];
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const bool keep = (f.idx != gi[k]) && (abs(f.row - cy[k]) <= 1);
+                        const double wx = f.x - xi[k], wy = f.y - yi[k];
+                        const double s2 = fma(wx, wx, wy * wy);
+                        double w;
+                        if (LAW == 0) w = f.m / (s2 + eps2);
+                        else w = (s2 > r2cut) ? f.m / (s2 + eps2) : 0.0;
+                        w = keep ? w : 0.0;
+                        sx[k] = fma(wx, w, sx[k]);
+                        sy[k] = fma(wy, w, sy[k]);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // far field: compact COM list of the system, minus the 3x3 blocks
+        const int ne0 = A.neprefix[coff], ne1 = A.neprefix[coff + (int64_t)G * G];
+        int mlo[3], mhi[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int xc = cx + t - 1;
+            if (xc >= 0 && xc < G) {
+                const int64_t cl = coff + (int64_t)xc * G;
+                mlo[t] = A.neprefix[cl + ylo];
+                mhi[t] = A.neprefix[cl + yhi + 1];
+            } else {
+                mlo[t] = 0;
+                mhi[t] = 0;
+            }
+        }
+        Com64 nxt;
+        {
+            const int e = ne0 + lane;
+            if (e < ne1) nxt = A.l64[e];
+            else { nxt.X = 1e150; nxt.Y = 1e150; nxt.M = 0.0; nxt.kx = -100000; nxt.ky = -100000; }
+        }
+        for (int base = ne0; base < ne1; base += BH_WAVE) {
+            ftile[lane] = nxt;
+            __syncthreads();
+            {
+                const int e = base + BH_WAVE + lane;
+                if (e < ne1) nxt = A.l64[e];
+                else { nxt.X = 1e150; nxt.Y = 1e150; nxt.M = 0.0; nxt.kx = -100000; nxt.ky = -100000; }
+            }
+            const int b1 = base + BH_WAVE;
+            const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||
+                                meets(base, b1, mlo[2], mhi[2]);
+            if (!masked) {
+#pragma unroll 4
+                for (int j = 0; j < BH_WAVE; ++j) {
+                    const Com64 f = ftile[j];
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const double wx = f.X - xi[k], wy = f.Y - yi[k];
+                        const double w = far_w64<LAW>(wx, wy, f.M, eps2, r2cut);
+                        sx[k] = fma(wx, w, sx[k]);
+                        sy[k] = fma(wy, w, sy[k]);
+                    }
+                }
+            } else {
+                for (int j = 0; j < BH_WAVE; ++j) {
+                    const Com64 f = ftile[j];
+                    const bool colnear = abs(f.kx - cx) <= 1;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const bool keep = !(colnear && abs(f.ky - cy[k]) <= 1);
+                        const double wx = f.X - xi[k], wy = f.Y - yi[k];
+                        double w = far_w64<LAW>(wx, wy, f.M, eps2, r2cut);
+                        w = keep ? w : 0.0;
+                        sx[k] = fma(wx, w, sx[k]);
+                        sy[k] = fma(wy, w, sy[k]);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (act[k]) {
+                A.ax[gi[k]] += A.Gc * sx[k];
+                A.ay[gi[k]] += A.Gc * sy[k];
+            }
+        }
+        if (A.cost_log && lane == 0) A.cost_log[itx] = (long long)wall_clock64() - t0;
+    }
+}
+
+// ---------------------------------------------------------------- FP32
+// Positions in cell units relative to the item origin (cx, oy), oy = the
+// item's middle row: the integer cell offsets are exact in FP32, so a far
+// term's separation carries a relative error of a few ulp.  Near-field
+// separations are formed in FP64 and rounded once.  KAHAN: 0 plain FP32
+// sums; 1 FP32 partial per 64-entry tile, tiles Kahan-summed; 2 Kahan per term.
+template <int P, int LAW, int KAHAN>
+__global__ __launch_bounds__(64) void k_force32(ForceArgs A)
+{
+    __shared__ float4 ftile[BH_WAVE];
+    __shared__ NearE ntile[BH_WAVE];
+    const int lane = threadIdx.x;
+    const int nitems = A.stats[ST_NITEMS];
+    for (;;) {
+        const int q = next_item(A.stats, lane);
+        if (q >= nitems) break;
+        const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
+        const int itx = A.order ? uniform_int(A.order[q]) : q;
+        if ((unsigned)itx >= (unsigned)nitems) continue;   // malformed order entry
+        const Item it = A.items[itx];
+        const SysDev& d = A.sys[it.sys];
+        const int G = d.G;
+        const int64_t coff = d.c_off;
+        const int cx = it.cx;
+        const int oy = (it.y0 + it.y1) >> 1;
+        const int ylo = max(it.y0 - 1, 0), yhi = min(it.y1 + 1, G - 1);
+        const double step = d.step;
+        const double inv_step = 1.0 / step;
+        const float eps2c = (float)(A.eps2 * inv_step * inv_step);
+        const float r2c = (float)(A.r2cut * inv_step * inv_step);
+
+        double xi[P], yi[P];
+        float ux[P], uy[P], Sx[P], Sy[P], Cx[P], Cy[P], tx[P], ty[P];
+        int cy[P], gi[P];
+        bool act[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            int slot = lane + BH_WAVE * k;
+            act[k] = slot < it.np;
+            gi[k] = it.p0 + slot;
+            xi[k] = 0.0;
+            yi[k] = 0.0;
+            ux[k] = 0.0f;
+            uy[k] = 0.0f;
+            cy[k] = -100000;
+            if (act[k]) {
+                xi[k] = A.x[gi[k]];
+                yi[k] = A.y[gi[k]];
+                cy[k] = (int)(A.key_sorted[gi[k]] - coff) - cx * G;
+                ux[k] = (float)((xi[k] - d.min_x) / step - (double)cx);
+                uy[k] = (float)((yi[k] - d.min_y) / step - (double)oy);
+            }
+            Sx[k] = Sy[k] = Cx[k] = Cy[k] = tx[k] = ty[k] = 0.0f;
+        }
+
+        // near field (separations in FP64, rounded once to FP32)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xc = cx + dx;
+            if (xc < 0 || xc >= G) continue;
+            const int64_t cl = coff + (int64_t)xc * G;
+            const int pa = A.offsets[cl + ylo], pb = A.offsets[cl + yhi + 1];
+            for (int base = pa; base < pb; base += BH_WAVE) {
+                const int j = base + lane;
+                NearE e;
+                if (j < pb) {
+                    e.x = A.x[j];
+                    e.y = A.y[j];
+                    e.m = A.m[j];
+                    e.row = (int)(A.key_sorted[j] - cl);
+                    e.idx = j;
+                } else {
+                    e.x = 0.0;
+                    e.y = 0.0;
+                    e.m = 0.0;
+                    e.row = -100000;
+                    e.idx = -1;
+                }
+                ntile[lane] = e;
+                __syncthreads();
+                const int cnt = min(BH_WAVE, pb - base);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const NearE f = ntile[jj];
+                    const float fm = (float)f.m;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const bool keep = (f.idx != gi[k]) && (abs(f.row - cy[k]) <= 1);
+                        const float wx = (float)((f.x - xi[k]) * inv_step);
+                        const float wy = (float)((f.y - yi[k]) * inv_step);
+                        const float s2 = fmaf(wx, wx, wy * wy);
+                        float t = fm * __builtin_amdgcn_rcpf(s2 + eps2c);
+                        if (LAW == 1) t = (s2 > r2c) ? t : 0.0f;
+                        t = keep ? t : 0.0f;
+                        if (KAHAN == 2) {
+                            kahan_add(Sx[k], Cx[k], wx * t);
+                            kahan_add(Sy[k], Cy[k], wy * t);
+                        } else {
+                            tx[k] = fmaf(wx, t, tx[k]);
+                            ty[k] = fmaf(wy, t, ty[k]);
+                        }
+                    }
+                }
+                if (KAHAN == 1) {
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        kahan_add(Sx[k], Cx[k], tx[k]);
+                        kahan_add(Sy[k], Cy[k], ty[k]);
+                        tx[k] = 0.0f;
+                        ty[k] = 0.0f;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+        // far field
+        const int ne0 = A.neprefix[coff], ne1 = A.neprefix[coff + (int64_t)G * G];
+        int mlo[3], mhi[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int xc = cx + t - 1;
+            if (xc >= 0 && xc < G) {
+                const int64_t cl = coff + (int64_t)xc * G;
+                mlo[t] = A.neprefix[cl + ylo];
+                mhi[t] = A.neprefix[cl + yhi + 1];
+            } else {
+                mlo[t] = 0;
+                mhi[t] = 0;
+            }
+        }
+        Com32 nxt;
+        {
+            const int e = ne0 + lane;
+            if (e < ne1) nxt = A.l32[e];
+            else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+        }
+        for (int base = ne0; base < ne1; base += BH_WAVE) {
+            {
+                const int kx = nxt.kxy >> 16, ky = nxt.kxy & 0xffff;
+                float4 v;
+                v.x = (float)(kx - cx) + nxt.fx;
+                v.y = (float)(ky - oy) + nxt.fy;
+                v.z = nxt.M;
+                v.w = __int_as_float(nxt.kxy);
+                ftile[lane] = v;
+            }
+            __syncthreads();
+            {
+                const int e = base + BH_WAVE + lane;
+                if (e < ne1) nxt = A.l32[e];
+                else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+            }
+            const int b1 = base + BH_WAVE;
+            const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||
+                                meets(base, b1, mlo[2], mhi[2]);
+            if (!masked) {
+#pragma unroll 8
+                for (int j = 0; j < BH_WAVE; ++j) {
+                    const float4 f = ftile[j];
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const float wx = f.x - ux[k], wy = f.y - uy[k];
+                        const float t = far_t32<LAW>(wx, wy, f.z, eps2c, r2c);
+                        if (KAHAN == 2) {
+                            kahan_add(Sx[k], Cx[k], wx * t);
+                            kahan_add(Sy[k], Cy[k], wy * t);
+                        } else {
+                            tx[k] = fmaf(wx, t, tx[k]);
+                            ty[k] = fmaf(wy, t, ty[k]);
+                        }
+                    }
+                }
+            } else {
+                for (int j = 0; j < BH_WAVE; ++j) {
+                    const float4 f = ftile[j];
+                    const int kxy = __float_as_int(f.w);
+                    const bool colnear = abs((kxy >> 16) - cx) <= 1;
+                    const int ky = kxy & 0xffff;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const bool keep = !(colnear && abs(ky - cy[k]) <= 1);
+                        const float wx = f.x - ux[k], wy = f.y - uy[k];
+                        float t = far_t32<LAW>(wx, wy, f.z, eps2c, r2c);
+                        t = keep ? t : 0.0f;
+                        if (KAHAN == 2) {
+                            kahan_add(Sx[k], Cx[k], wx * t);
+                            kahan_add(Sy[k], Cy[k], wy * t);
+                        } else {
+                            tx[k] = fmaf(wx, t, tx[k]);
+                            ty[k] = fmaf(wy, t, ty[k]);
+                        }
+                    }
+                }
+            }
+            if (KAHAN == 1) {
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    kahan_add(Sx[k], Cx[k], tx[k]);
+                    kahan_add(Sy[k], Cy[k], ty[k]);
+                    tx[k] = 0.0f;
+                    ty[k] = 0.0f;
+                }
+            }
+            __syncthreads();
+        }
+
+        const double scale = A.Gc * inv_step;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (act[k]) {
+                double rx, ry;
+                if (KAHAN == 0) {
+                    rx = (double)tx[k];
+                    ry = (double)ty[k];
+                } else {
+                    rx = (double)Sx[k] - (double)Cx[k];
+                    ry = (double)Sy[k] - (double)Cy[k];
+                }
+                A.ax[gi[k]] += scale * rx;
+                A.ay[gi[k]] += scale * ry;
+            }
+        }
+        if (A.cost_log && lane == 0) A.cost_log[itx] = (long long)wall_clock64() - t0;
+    }
+}
+
+// ---------------------------------------------------------------- dispatch table
+typedef void (*force_fn)(ForceArgs);
+
+#define F64(P, L) (force_fn)k_force64<P, L>
+#define F32(P, L, K) (force_fn)k_force32<P, L, K>
+
+// [precision: 0 fp64, 1..3 fp32 kahan mode 1,2,0][P index][law]
+static const force_fn g_force[4][3][2] = {
+    {{F64(1, 0), F64(1, 1)}, {F64(2, 0), F64(2, 1)}, {F64(4, 0), F64(4, 1)}},
+    {{F32(1, 0, 1), F32(1, 1, 1)}, {F32(2, 0, 1), F32(2, 1, 1)}, {F32(4, 0, 1), F32(4, 1, 1)}},
+    {{F32(1, 0, 2), F32(1, 1, 2)}, {F32(2, 0, 2), F32(2, 1, 2)}, {F32(4, 0, 2), F32(4, 1, 2)}},
+    {{F32(1, 0, 0), F32(1, 1, 0)}, {F32(2, 0, 0), F32(2, 1, 0)}, {F32(4, 0, 0), F32(4, 1, 0)}},
+};
+
+const void* bh_force_kernel(int precision, int pidx, int law)
+{
+    if (precision < 0 || precision > 3 || pidx < 0 || pidx > 2 || law < 0 || law > 1) return nullptr;
+    return (const void*)g_force[precision][pidx][law];
+}

@@ -1,0 +1,78 @@
+// Kernel declarations and tile constants shared by the .hip translation units.
+#pragma once
+#include "bh_common.h"
+
+#define PLACE_CHUNK 2048   // particles per block in bbox / keys / scatter
+#define HIST_BINS 8192     // LDS histogram window (keys) per block
+#define SCAN_TILE 2048     // elements per block in the scans
+#define SORT_W 2048        // within-cell rank window (rows)
+
+struct SortFields {
+    const double *px, *py, *m, *vx, *vy, *ax, *ay;
+    const int32_t* pid;
+};
+
+// Arguments of the force kernels (passed by value).
+struct ForceArgs {
+    const Item* items;
+    const int32_t* order;      // queue position -> item (NULL: identity)
+    int32_t* stats;
+    const SysDev* sys;
+    const int32_t* offsets;    // cell -> first sorted particle (C + 1)
+    const int32_t* neprefix;   // cell -> first compact non-empty index (C + 1)
+    const int32_t* key_sorted; // sorted particle -> global cell
+    const double *x, *y, *m;
+    const Com64* l64;
+    const Com32* l32;
+    double *ax, *ay;
+    double Gc, eps2, r2cut;
+    long long* cost_log;       // per item duration (wall clock ticks) or NULL
+};
+
+// placement
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
+__global__ void k_fill_f64(double* p, int64_t n, double v);
+__global__ void k_bbox_init(uint64_t* bb, int S);
+__global__ void k_bbox(const double* px, const double* py, const int32_t* sys_of, int64_t N,
+                       uint64_t* bb, int32_t* stats);
+__global__ void k_bbox_finalize(SysDev* sys, const uint64_t* bb, int S, int32_t* stats);
+__global__ void k_keys_hist(const double* px, const double* py, const int32_t* sys_of,
+                            const SysDev* sys, int64_t N, int32_t* key_out, int32_t* counts);
+template <bool CNT>
+__global__ void k_scan_reduce(const int32_t* in, int64_t n, int32_t* bsum, int32_t* bne,
+                              int32_t* stats);
+template <bool CNT>
+__global__ void k_scan_blocks(int32_t* bsum, int32_t* bne, int64_t nb, int32_t* stats,
+                              int total_slot);
+template <bool CNT>
+__global__ void k_scan_final(const int32_t* in, int64_t n, const int32_t* bsum, const int32_t* bne,
+                             int32_t* out_v, int32_t* out_f);
+__global__ void k_scatter(const int32_t* key, int64_t N, const int32_t* offsets, int32_t* cursor,
+                          int32_t* idx_out, int32_t* key_sorted);
+__global__ void k_sort_units(SortFields F, int64_t N, const int32_t* idx_in,
+                             const int32_t* key_sorted, const int32_t* offsets, int32_t* idx_out);
+__global__ void k_sort_merge(SortFields F, int64_t N, const int32_t* idx_in,
+                             const int32_t* key_sorted, const int32_t* offsets, int32_t* idx_out,
+                             int L, const int32_t* stats);
+__global__ void k_gather(SortFields S, BhParticles D, int64_t N, const int32_t* idx_a,
+                         const int32_t* idx_b, const int32_t* key_sorted, const int32_t* offsets,
+                         const int32_t* stats, int32_t* perm_out);
+
+// summarize + tick
+__global__ void k_ne_list(const int32_t* counts, const int32_t* neprefix, int64_t C,
+                          int32_t* ne_list);
+__global__ void k_com(const SysDev* sys, int S, const int32_t* ne_list, const int32_t* stats,
+                      const int32_t* offsets, const double* x, const double* y, const double* m,
+                      double* com_dense, Com64* l64, Com32* l32);
+__global__ void k_tick(double* x, double* y, double* vx, double* vy, double* ax, double* ay,
+                       int64_t N, double dt);
+__global__ void k_interactions(const SysDev* sys, int S, const int32_t* counts,
+                               const int32_t* neprefix, int64_t C,
+                               unsigned long long* out /* 2*S + 2 */);
+
+// force
+__global__ void k_clamp_items(int32_t* stats, int64_t max_items);
+template <bool WRITE>
+__global__ void k_items(const SysDev* sys, int S, int64_t ncols, const int32_t* counts,
+                        const int32_t* offsets, int pitem, int yspan, int32_t* col_items,
+                        Item* items, int64_t max_items, int32_t* stats);

@@ -1,0 +1,150 @@
+// Summarize (per-cell centre of mass), tick, and the exact interaction
+// counter used for work accounting.
+//   g_summarize       python-bh/barneshut/kernels/grid_decomposition/gpu/grid.py:78-114
+//   calc_COM          python-bh/barneshut/internals/cloud.py:9-17
+//   g_tick_particles  python-bh/barneshut/kernels/grid_decomposition/gpu/grid.py:159-174
+#include "bh_common.h"
+#include "bh_kernels.h"
+
+// Compact list of non-empty cells in global cell order.
+__global__ void k_ne_list(const int32_t* __restrict__ counts, const int32_t* __restrict__ neprefix,
+                          int64_t C, int32_t* __restrict__ ne_list)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += stride)
+        if (counts[c] > 0) ne_list[neprefix[c]] = (int32_t)c;
+}
+
+// One wave per non-empty cell: FP64 sums of m*x, m*y, m over the cell's
+// sorted particles, butterfly-reduced (fixed order, deterministic).
+// Writes the dense G*G x 3 array of the reference layout (empty cells were
+// zero-filled) and the two compact lists the force kernels stream.
+__global__ __launch_bounds__(256) void k_com(const SysDev* __restrict__ sys, int S,
+                                             const int32_t* __restrict__ ne_list,
+                                             const int32_t* __restrict__ stats,
+                                             const int32_t* __restrict__ offsets,
+                                             const double* __restrict__ x,
+                                             const double* __restrict__ y,
+                                             const double* __restrict__ m,
+                                             double* __restrict__ com_dense,
+                                             Com64* __restrict__ l64, Com32* __restrict__ l32)
+{
+    const int lane = threadIdx.x & 63;
+    int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int nne = stats[ST_NCELLS_NE];
+    for (; w < nne; w += nw) {
+        int c = ne_list[w];
+        int a = offsets[c], b = offsets[c + 1];
+        double sx = 0.0, sy = 0.0, sm = 0.0;
+        for (int i = a + lane; i < b; i += BH_WAVE) {
+            double mi = m[i];
+            sx = fma(x[i], mi, sx);
+            sy = fma(y[i], mi, sy);
+            sm += mi;
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sm = wave_sum(sm);
+        if (lane == 0) {
+            double X = 0.0, Y = 0.0, M = 0.0;
+            if (sm != 0.0) {
+                X = sx / sm;
+                Y = sy / sm;
+                M = sm;
+            }
+            com_dense[3 * (int64_t)c + 0] = X;
+            com_dense[3 * (int64_t)c + 1] = Y;
+            com_dense[3 * (int64_t)c + 2] = M;
+            const SysDev& d = sys[sys_of_cell(sys, S, c)];
+            int local = (int)(c - d.c_off);
+            int kx = local / d.G, ky = local - kx * d.G;
+            Com64 e64;
+            e64.X = X;
+            e64.Y = Y;
+            e64.M = M;
+            e64.kx = kx;
+            e64.ky = ky;
+            l64[w] = e64;
+            Com32 e32;
+            e32.fx = (float)((X - d.min_x) / d.step - (double)kx);
+            e32.fy = (float)((Y - d.min_y) / d.step - (double)ky);
+            e32.M = (float)M;
+            e32.kxy = (kx << 16) | ky;
+            l32[w] = e32;
+        }
+    }
+}
+
+// v += a*dt; a = 0; x += v*dt  (separate multiply and add; the library is
+// built with -ffp-contract=off).
+__global__ __launch_bounds__(256) void k_tick(double* __restrict__ x, double* __restrict__ y,
+                                              double* __restrict__ vx, double* __restrict__ vy,
+                                              double* __restrict__ ax, double* __restrict__ ay,
+                                              int64_t N, double dt)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        double vxi = vx[i] + ax[i] * dt;
+        double vyi = vy[i] + ay[i] * dt;
+        vx[i] = vxi;
+        vy[i] = vyi;
+        ax[i] = 0.0;
+        ay[i] = 0.0;
+        x[i] = x[i] + vxi * dt;
+        y[i] = y[i] + vyi * dt;
+    }
+}
+
+// Exact interaction counts of one evaluation, per system and in total:
+//   near_s = sum_c n_c * sum_{3x3} n_c'   (the host subtracts N_s)
+//   far_s  = sum_c n_c * (K_s - #non-empty in 3x3)
+__global__ __launch_bounds__(256) void k_interactions(const SysDev* __restrict__ sys, int S,
+                                                      const int32_t* __restrict__ counts,
+                                                      const int32_t* __restrict__ neprefix,
+                                                      int64_t C, unsigned long long* out)
+{
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < C; c0 += stride) {
+        int64_t c = c0 + threadIdx.x;
+        unsigned long long nr = 0, fr = 0;
+        int s = -1;
+        if (c < C && counts[c] > 0) {
+            s = sys_of_cell(sys, S, c);
+            const SysDev& d = sys[s];
+            int G = d.G;
+            int local = (int)(c - d.c_off);
+            int cx = local / G, cy = local - cx * G;
+            long long n9 = 0, k9 = 0;
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    int nx = cx + dx, ny = cy + dy;
+                    if (nx < 0 || ny < 0 || nx >= G || ny >= G) continue;
+                    int cc = counts[d.c_off + (int64_t)nx * G + ny];
+                    n9 += cc;
+                    k9 += cc > 0;
+                }
+            long long K = neprefix[d.c_off + (int64_t)G * G] - neprefix[d.c_off];
+            long long nc = counts[c];
+            nr = (unsigned long long)(nc * n9);
+            fr = (unsigned long long)(nc * (K - k9));
+        }
+        int s0 = __shfl(s, 0, BH_WAVE);
+        bool uni = __all(s == s0 || s < 0);
+        if (uni) {
+            int sv = wave_max(s);
+            unsigned long long a = wave_sum(nr), b = wave_sum(fr);
+            if ((threadIdx.x & 63) == 0 && sv >= 0) {
+                atomicAdd(&out[2 * sv], a);
+                atomicAdd(&out[2 * sv + 1], b);
+                atomicAdd(&out[2 * S], a);
+                atomicAdd(&out[2 * S + 1], b);
+            }
+        } else if (s >= 0) {
+            atomicAdd(&out[2 * s], nr);
+            atomicAdd(&out[2 * s + 1], fr);
+            atomicAdd(&out[2 * S], nr);
+            atomicAdd(&out[2 * S + 1], fr);
+        }
+    }
+}

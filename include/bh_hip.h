@@ -1,0 +1,164 @@
+/*
+ * bh_hip.h -- C ABI of the MI355X (gfx950) grid-method step library
+ * (libbh_hip.so).  Plain pointers and sizes only; all particle buffers are
+ * device pointers owned by the caller, scratch and per-step state are owned
+ * by the context.  Every call returns BH_OK (0) or an error code; the text of
+ * the last error of the calling thread is bh_last_error().  Launches are
+ * asynchronous on the given stream (a hipStream_t, NULL = default stream).
+ * A context is used by one host thread at a time.
+ *
+ * Reference interfaces replaced (python-bh/...):
+ *   bh_place_particles     set_particles_bounding_box  barneshut/implementations/base.py:211-239
+ *                          g_place_particles          barneshut/kernels/grid_decomposition/gpu/grid.py:15-50
+ *   bh_calculate_box_cumm  g_calculate_box_cumm       .../gpu/grid.py:52-60
+ *   bh_sort_particles      g_sort_particles           .../gpu/grid.py:62-76 (order of sequential.py:37)
+ *   bh_summarize           g_summarize                .../gpu/grid.py:78-114
+ *   bh_evaluate_boxes      g_evaluate_boxes           .../gpu/grid.py:116-156
+ *   bh_tick_particles      g_tick_particles           .../gpu/grid.py:159-174
+ *   bh_step / bh_step_graph  one BaseBarnesHut.run round, base.py:94-135
+ *   bh_create/bh_set_systems  SingleGPUBarnesHut.__init_device_arrays, singlegpu.py:28-39
+ */
+#ifndef BH_HIP_H
+#define BH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BH_OK = 0,
+    BH_EINVAL = 1,   /* bad argument */
+    BH_EHIP = 2,     /* HIP runtime error */
+    BH_EDEVICE = 3,  /* a kernel raised an error flag (see bh_read_stats) */
+    BH_ESTATE = 4    /* call made before bh_set_systems */
+};
+
+/* Device-side error bits reported by bh_read_stats()[0]. */
+#define BH_DEVERR_NONFINITE 1   /* NaN/inf particle coordinate */
+#define BH_DEVERR_DEGENERATE 2  /* zero-extent bounding box */
+#define BH_DEVERR_ITEMS 4       /* work-item capacity exceeded */
+
+enum {
+    BH_PREC_FP64 = 0,            /* plain FP64 forces */
+    BH_PREC_FP32_KAHAN = 1,      /* FP32 cell-relative, Kahan-summed 64-term tiles */
+    BH_PREC_FP32_KAHAN_TERM = 2, /* FP32 cell-relative, Kahan per term */
+    BH_PREC_FP32_PLAIN = 3       /* FP32 cell-relative, uncompensated */
+};
+
+/* SoA particle buffers (device), length = total particles of all systems. */
+typedef struct bh_particles {
+    double *px, *py, *mass, *vx, *vy, *ax, *ay;
+    int32_t* pid;
+} bh_particles;
+
+/* a_i += G * sum m_j (x_j - x_i) * w(r2),  w(s) = 1/(s + eps2), and
+ * w(s) = 0 for s <= r2_cut when r2_cut > 0.
+ * GPU reference: eps2 = 0, r2_cut = 0; CPU reference: r2_cut = 1e-5. */
+typedef struct bh_law {
+    double G;
+    double eps2;
+    double r2_cut;
+} bh_law;
+
+/* Launch shape of the force kernel. p: particles per lane (1, 2 or 4);
+ * yspan: max rows per work item (>= 1); waves_per_cu: persistent waves per
+ * compute unit (>= 1). */
+typedef struct bh_tuning {
+    int32_t p;
+    int32_t yspan;
+    int32_t waves_per_cu;
+} bh_tuning;
+
+/* What bh_export copies out. */
+enum {
+    BH_EXPORT_COUNTS = 0,     /* int32 [n_cells]   particles per global cell          */
+    BH_EXPORT_OFFSETS = 1,    /* int32 [n_cells+1] first sorted particle of each cell */
+    BH_EXPORT_NEPREFIX = 2,   /* int32 [n_cells+1] compact index of each cell         */
+    BH_EXPORT_PERM = 3,       /* int32 [n]         sorted position -> source row      */
+    BH_EXPORT_KEYS = 4,       /* int32 [n]         global cell of each sorted row     */
+    BH_EXPORT_COM = 5,        /* double [n_cells*3] X, Y, M per cell (0 if empty)     */
+    BH_EXPORT_BBOX = 6,       /* double [5*S] min_x, min_y, max_x, max_y, step        */
+    BH_EXPORT_ITEMS = 7,      /* int32 [n_items*8] p0, np, sys, cx, y0, y1, 0, 0      */
+    BH_EXPORT_COST = 8,       /* int64 [n_items]   per-item wall-clock ticks (100 MHz)*/
+    BH_EXPORT_KEYS_IN = 9     /* int32 [n]         global cell of each input row      */
+};
+
+const char* bh_last_error(void);
+int bh_version(void);
+
+int bh_create(int device, void** ctx_out);
+int bh_destroy(void* ctx);
+
+/* Defines the independent systems (host arrays of length nsys): system s
+ * holds n[s] particles at rows [sum_{t<s} n[t], ...) and a grid_dim[s]^2
+ * grid. (Re)allocates the context's scratch. */
+int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* grid_dim);
+int bh_set_tuning(void* ctx, const bh_tuning* t);
+int bh_get_tuning(void* ctx, bh_tuning* t);
+/* 1: the force kernel logs each work item's duration (BH_EXPORT_COST). */
+int bh_set_cost_log(void* ctx, int32_t enable);
+
+int bh_place_particles(void* ctx, const bh_particles* p, void* stream);
+int bh_calculate_box_cumm(void* ctx, void* stream);
+int bh_sort_particles(void* ctx, const bh_particles* src, const bh_particles* dst, void* stream);
+int bh_summarize(void* ctx, const bh_particles* p, void* stream);
+/* order: device int32 [n_items] queue order of the work items, or NULL for
+ * the static (natural) order; it must come from this step's items. */
+int bh_build_items(void* ctx, void* stream);
+int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
+                      const int32_t* order, void* stream);
+int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream);
+
+/* One full round: a is placed and sorted into b, then summarize, evaluate
+ * (order NULL: static schedule) and, if do_tick, tick -- all on b. */
+int bh_step(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+            int32_t precision, double dt, int32_t do_tick, void* stream);
+/* Same as bh_step, replayed from a captured HIP graph (captured on first use
+ * for a given (a, b, law, precision, dt, do_tick, tuning)). */
+int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+                  int32_t precision, double dt, int32_t do_tick, void* stream);
+
+/* Synchronises the stream, copies the int32 stats block (16 entries:
+ * [0] error bits, [1] max cell count, [2] work items, [4] non-empty cells)
+ * to host memory out16, and returns BH_EDEVICE if an error bit is set. */
+int bh_read_stats(void* ctx, int32_t* out16, void* stream);
+/* Copies state `what` into dst (device or host memory), count elements. */
+int bh_export(void* ctx, int32_t what, void* dst, int64_t count, void* stream);
+/* Exact interaction counts of the current placement: host out[2*S+2] =
+ * (near_s, far_s) per system, then the totals. Synchronises. */
+int bh_interactions(void* ctx, int64_t* out, void* stream);
+
+/* Learned cost model (MLP 8 -> 32 -> 32 -> 1, FP32, ReLU).
+ * params: device float[BH_MLP_NPARAM] = W1[32x8], b1[32], W2[32x32], b2[32],
+ * W3[32], b3[1] (row-major, out x in). */
+#define BH_MLP_NPARAM 1377
+#define BH_MLP_NFEAT 8
+/* Device features [n_items x 8] of the current work items (build_items first). */
+int bh_item_features(void* ctx, float* feat, void* stream);
+int bh_mlp_forward(const float* params, const float* x, int64_t n, float* y, void* stream);
+/* Mean-squared-error gradient over n rows: grad[BH_MLP_NPARAM] = d/dparams
+ * of mean((y - target)^2); loss (device float[1]) optional. */
+int bh_mlp_backward(const float* params, const float* x, const float* target, int64_t n,
+                    float* grad, float* loss, void* stream);
+/* Adam update: m, v moments (device float[BH_MLP_NPARAM]), step t >= 1. */
+int bh_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t t, float lr,
+                void* stream);
+/* Queue order from predicted costs (device float[n_items]): longest first,
+ * by 4096 buckets between the smallest and largest prediction (order inside
+ * a bucket is unspecified). Writes device int32 order[n_items]. A non-finite
+ * prediction makes the whole order fall back to the static order. */
+int bh_order_by_cost(void* ctx, const float* cost, int32_t* order, void* stream);
+/* Features -> MLP forward -> bh_order_by_cost for the current items, all on
+ * the device (no host sync; capturable). */
+int bh_learned_order(void* ctx, const float* params, int32_t* order, void* stream);
+
+/* Context sizes: what = 0 particles, 1 cells, 2 systems, 3 columns,
+ * 4 work-item capacity, 5 compute units. */
+int bh_query(void* ctx, int32_t what, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BH_HIP_H */

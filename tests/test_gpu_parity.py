@@ -1,0 +1,335 @@
+"""GPU parity of the HIP step against the CPU oracle (through the C ABI).
+
+Placement (bounding box, cell keys, counts, offsets, permutation, gathered
+rows) and the tick are compared bit for bit; COMs and forces within the
+tolerances stated in each test.  Sizes are small enough for the oracle to
+finish in seconds.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+from oracle import oracle as orc  # noqa: E402
+from barneshut import inputs  # noqa: E402
+
+
+def _engine(n, G, tuning=None):
+    from barneshut.engine import Engine
+    return Engine([n], [G], tuning=tuning)
+
+
+def _ties(n, seed=3):
+    """Clustered set with exact ties: clipped rows at 0 and duplicated px."""
+    d = inputs.gaussian_mixture(n, 8, sigma=(0.02, 0.3), seed=seed)
+    rng = np.random.default_rng(seed)
+    k = n // 10
+    idx = rng.choice(n, k, replace=False)
+    d["px"][idx[: k // 2]] = d["px"][idx[k // 2:k // 2 * 2]]   # equal px, different py
+    j = rng.choice(n, 20, replace=False)
+    d["py"][j[:10]] = d["py"][j[10:]]
+    d["px"][j[:10]] = d["px"][j[10:]]                     # identical positions
+    d["mass"][j[:5]] = d["mass"][j[10:15]]                  # identical up to pid
+    return d
+
+
+CASES = [
+    ("c1_uniform", lambda: inputs.uniform(16384, seed=1), 32),
+    ("ref_normal", lambda: inputs.reference_normal(20000, seed=2), orc.grid_dim(20000, 1000)),
+    ("ties_G24", lambda: _ties(30000), 24),
+    ("big_cells_G2", lambda: inputs.gaussian_mixture(9000, 3, (0.05, 0.2), seed=4), 2),
+    ("one_cell_G1", lambda: inputs.uniform(5000, seed=5), 1),
+]
+
+
+def _oracle_placement(d, G):
+    n = len(d["px"])
+    bb = orc.bbox(d["px"], d["py"], G)
+    gx, gy, key = orc.place(d["px"], d["py"], bb[0], bb[1], bb[4], G)
+    z = np.zeros(n)
+    perm = orc.sort_perm(key, (d["px"], d["py"], d["mass"], d["vx"], d["vy"], z, z,
+                               d["pid"].astype(np.float64)))
+    cnt, off = orc.counts_offsets(key[perm], G * G)
+    return bb, key, perm, cnt, off
+
+
+@pytest.mark.parametrize("name,make,G", CASES, ids=[c[0] for c in CASES])
+def test_placement_bit_exact(name, make, G):
+    d = make()
+    print(f"[placement] case={name} n={len(d['px'])} G={G}", flush=True)
+    eng = _engine(len(d["px"]), G)
+    eng.load(d)
+    eng.create_tree()
+    eng.check()
+    bb, key, perm, cnt, off = _oracle_placement(d, G)
+    gbb = eng.bbox()[0]
+    assert gbb[0] == bb[0] and gbb[1] == bb[1] and gbb[2] == bb[2] and gbb[4] == bb[4]
+    np.testing.assert_array_equal(eng.keys_in(), key)
+    np.testing.assert_array_equal(eng.counts(), cnt)
+    np.testing.assert_array_equal(eng.offsets(), off)
+    np.testing.assert_array_equal(eng.perm(), perm)
+    h = eng.fetch()
+    for k in ("px", "py", "mass", "vx", "vy", "pid"):
+        np.testing.assert_array_equal(h[k], np.asarray(d[k])[perm])
+    np.testing.assert_array_equal(eng.keys(), key[perm])
+    eng.close()
+
+
+def test_placement_matches_numpy_structured_sort():
+    """Same permutation as the reference's own numpy call (sequential.py:37)."""
+    d = _ties(12000, seed=7)
+    G = 16
+    n = len(d["px"])
+    aos = np.zeros((n, 10))
+    for c, k in enumerate(("px", "py", "mass", "vx", "vy")):
+        aos[:, c] = d[k]
+    aos[:, 9] = d["pid"]
+    srt, (mnx, mny, step) = orc.place_numpy(aos, G)
+    eng = _engine(n, G)
+    eng.load(d)
+    eng.create_tree()
+    h = eng.fetch()
+    np.testing.assert_array_equal(h["px"], srt[:, 0])
+    np.testing.assert_array_equal(h["py"], srt[:, 1])
+    np.testing.assert_array_equal(h["pid"], srt[:, 9].astype(np.int32))
+    keys = eng.keys()
+    np.testing.assert_array_equal(keys // G, srt[:, 7].astype(np.int32))
+    np.testing.assert_array_equal(keys % G, srt[:, 8].astype(np.int32))
+    eng.close()
+
+
+def _forces_setup(d, G, tuning=None):
+    eng = _engine(len(d["px"]), G, tuning)
+    eng.load(d)
+    eng.create_tree()
+    eng.summarize()
+    return eng
+
+
+def _oracle_forces(eng, G, Gc, eps2, r2cut):
+    h = eng.fetch()
+    off = eng.offsets().astype(np.int64)
+    keys = eng.keys()
+    com = orc.summarize(off, h["px"], h["py"], h["mass"])
+    ax, ay = orc.forces(G, off, keys, h["px"], h["py"], h["mass"], com, Gc, eps2, r2cut)
+    return h, com, ax, ay
+
+
+def _rel(ax, ay, rx, ry):
+    num = np.hypot(ax - rx, ay - ry)
+    den = np.hypot(rx, ry)
+    return float(np.sqrt((num ** 2).sum() / (den ** 2).sum())), num / den
+
+
+def test_com_fp64():
+    d = inputs.gaussian_mixture(40000, 16, (0.005, 0.05), seed=11)
+    G = 64
+    eng = _forces_setup(d, G)
+    h = eng.fetch()
+    off = eng.offsets().astype(np.int64)
+    ref = orc.summarize(off, h["px"], h["py"], h["mass"])
+    com = eng.com()
+    # FP64 sums of <= a few hundred terms: agreement to ~1e-15 relative
+    np.testing.assert_allclose(com[:, 2], ref[:, 2], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(com[:, :2], ref[:, :2], rtol=1e-13, atol=1e-15)
+    eng.close()
+
+
+LAWS = [("gpu_ref", 1.0, 0.0, 0.0), ("cpu_ref_cut", 1.0, 0.0, 1e-5),
+        ("c1_soft", 1.0, 1e-6, 0.0), ("G2_soft_cut", 2.5, 1e-6, 1e-5)]
+
+
+@pytest.mark.parametrize("law", LAWS, ids=[l[0] for l in LAWS])
+def test_forces_fp64(law):
+    _, Gc, eps2, r2 = law
+    d = inputs.uniform(16384, seed=21)
+    G = 32
+    eng = _forces_setup(d, G)
+    eng.evaluate(Gc, eps2, r2, "fp64")
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, Gc, eps2, r2)
+    g = eng.fetch()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    print(f"[fp64 {law[0]}] normwise={norm:.3e} max={per.max():.3e}", flush=True)
+    assert norm <= 1e-12
+    assert per.max() <= 1e-9
+    eng.close()
+
+
+@pytest.mark.parametrize("prec,tol_norm,tol_p99", [("fp32", 1e-5, 1e-5),
+                                                   ("fp32_kahan_term", 1e-5, 1e-5),
+                                                   ("fp32_plain", 1e-4, 1e-4)])
+def test_forces_fp32(prec, tol_norm, tol_p99):
+    d = inputs.gaussian_mixture(60000, 16, (0.005, 0.05), seed=31)
+    G = 128
+    eng = _forces_setup(d, G)
+    eng.evaluate(1.0, 1e-8, 0.0, prec)
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, 1.0, 1e-8, 0.0)
+    g = eng.fetch()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    p99 = float(np.quantile(per, 0.99))
+    print(f"[{prec}] normwise={norm:.3e} p99={p99:.3e} max={per.max():.3e}", flush=True)
+    assert norm <= tol_norm
+    assert p99 <= tol_p99
+    eng.close()
+
+
+@pytest.mark.parametrize("G", [1, 2])
+def test_small_grid_is_direct_sum(G):
+    """G <= 2: every cell is a neighbour, so the method is the direct sum."""
+    d = inputs.uniform(3000, seed=40 + G)
+    eng = _forces_setup(d, G)
+    eng.evaluate(1.0, 1e-6, 0.0, "fp64")
+    g = eng.fetch()
+    rx, ry = orc.direct(g["px"], g["py"], g["mass"], 1.0, 1e-6, 0.0)
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    assert norm <= 1e-12 and per.max() <= 1e-9
+    eng.close()
+
+
+def test_single_particle_cells_far_field_is_direct_sum():
+    """Every non-empty cell holds one particle: COM terms equal particle terms."""
+    G = 64
+    rng = np.random.default_rng(5)
+    cells = rng.choice(np.arange(1, G * G - 1), 700, replace=False)
+    x = (cells // G + 0.5) / G
+    y = (cells % G + 0.5) / G
+    x[0], y[0] = 0.0, 0.0      # bbox [0, 1]^2 -> step = 1/G exactly
+    x[1], y[1] = 1.0, 1.0
+    d = inputs._pack(x, y, rng.uniform(1, 2, len(x)))
+    eng = _forces_setup(d, G)
+    assert eng.counts().max() == 1
+    eng.evaluate(1.0, 0.0, 0.0, "fp64")
+    g = eng.fetch()
+    rx, ry = orc.direct(g["px"], g["py"], g["mass"], 1.0, 0.0, 0.0)
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    assert norm <= 1e-12 and per.max() <= 1e-9
+    eng.close()
+
+
+def test_tick_bit_exact():
+    d = inputs.reference_normal(10000, seed=8)
+    G = 16
+    eng = _forces_setup(d, G)
+    eng.evaluate(1.0, 0.0, 1e-5, "fp64")
+    pre = eng.fetch()
+    eng.timestep(0.005)
+    post = eng.fetch()
+    x, y = pre["px"].copy(), pre["py"].copy()
+    vx, vy = pre["vx"].copy(), pre["vy"].copy()
+    ax, ay = pre["ax"].copy(), pre["ay"].copy()
+    orc.tick(x, y, vx, vy, ax, ay, 0.005)
+    for k, ref in (("px", x), ("py", y), ("vx", vx), ("vy", vy), ("ax", ax), ("ay", ay)):
+        np.testing.assert_array_equal(post[k], ref)
+    eng.close()
+
+
+def test_fp64_forces_independent_of_tuning():
+    """FP64 sums run in a fixed order per particle: bit-identical across
+    launch shapes and repeated runs."""
+    d = inputs.gaussian_mixture(30000, 8, (0.01, 0.05), seed=9)
+    G = 64
+    ref = None
+    for tun in ({"p": 1, "yspan": 1, "waves_per_cu": 4}, {"p": 2, "yspan": 16, "waves_per_cu": 16},
+                {"p": 4, "yspan": 64, "waves_per_cu": 8}, {"p": 2, "yspan": 16, "waves_per_cu": 16}):
+        eng = _forces_setup(d, G, tun)
+        eng.evaluate(1.0, 1e-6, 0.0, "fp64")
+        g = eng.fetch()
+        if ref is None:
+            ref = g
+        else:
+            np.testing.assert_array_equal(g["ax"], ref["ax"])
+            np.testing.assert_array_equal(g["ay"], ref["ay"])
+        eng.close()
+
+
+def test_step_and_graph_match_phases():
+    """bh_step and its graph replay equal the phase-by-phase calls, over
+    several consecutive steps."""
+    d = inputs.gaussian_mixture(20000, 8, (0.01, 0.05), seed=12)
+    G = 48
+    kw = dict(G=1.0, eps2=1e-6, r2_cut=0.0, precision="fp64", dt=1e-4)
+    e1 = _engine(len(d["px"]), G)
+    e2 = _engine(len(d["px"]), G)
+    e3 = _engine(len(d["px"]), G)
+    for e in (e1, e2, e3):
+        e.load(d)
+    for _ in range(4):
+        e1.create_tree()
+        e1.summarize()
+        e1.evaluate(kw["G"], kw["eps2"], kw["r2_cut"], kw["precision"])
+        e1.timestep(kw["dt"])
+        e2.step(**kw)
+        e3.step(graph=True, **kw)
+        a, b, c = e1.fetch(), e2.fetch(), e3.fetch()
+        for k in ("px", "py", "vx", "vy", "pid"):
+            np.testing.assert_array_equal(a[k], b[k])
+            np.testing.assert_array_equal(a[k], c[k])
+    for e in (e1, e2, e3):
+        e.close()
+
+
+def test_multi_step_against_oracle():
+    """Three full steps (place, summarize, evaluate, tick) against the oracle
+    step: placement exact at every step, positions within FP64 tolerance."""
+    d = inputs.uniform(8192, seed=13)
+    G = 32
+    eng = _engine(len(d["px"]), G)
+    eng.load(d)
+    st = dict(d)
+    st["ax"] = np.zeros(8192)
+    st["ay"] = np.zeros(8192)
+    for s in range(3):
+        out = orc.step(st["px"], st["py"], st["mass"], st["vx"], st["vy"], st["pid"], G, 1.0,
+                       1e-3, 1e-6, 0.0, st["ax"], st["ay"])
+        eng.step(G=1.0, eps2=1e-6, r2_cut=0.0, precision="fp64", dt=1e-3)
+        g = eng.fetch()
+        np.testing.assert_array_equal(g["pid"], out["pid"])
+        np.testing.assert_allclose(g["px"], out["px_next"], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(g["vx"], out["vx_next"], rtol=1e-9, atol=1e-12)
+        # continue both from the GPU state so placement parity is exact
+        st = dict(px=g["px"], py=g["py"], mass=g["mass"], vx=g["vx"], vy=g["vy"], pid=g["pid"],
+                  ax=g["ax"], ay=g["ay"])
+    eng.close()
+
+
+def test_batched_systems_match_single():
+    """Independent systems in one context give each system's own result."""
+    from barneshut.engine import Engine
+    sizes = [3000, 1024, 7000, 2500]
+    grids = [14, 8, 21, 13]
+    parts = [inputs.uniform(n, seed=50 + i) for i, n in enumerate(sizes)]
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    eng = Engine(sizes, grids)
+    eng.load(cat)
+    eng.create_tree()
+    eng.summarize()
+    eng.evaluate(1.0, 1e-6, 0.0, "fp64")
+    eng.check()
+    g = eng.fetch()
+    off = 0
+    for i, (n, G) in enumerate(zip(sizes, grids)):
+        sl = slice(off, off + n)
+        bb, key, perm, cnt, offs = _oracle_placement(parts[i], G)
+        np.testing.assert_array_equal(g["pid"][sl], parts[i]["pid"][perm])
+        com = orc.summarize(offs, g["px"][sl], g["py"][sl], g["mass"][sl])
+        rx, ry = orc.forces(G, offs, key[perm], g["px"][sl], g["py"][sl], g["mass"][sl], com,
+                            1.0, 1e-6, 0.0)
+        norm, per = _rel(g["ax"][sl], g["ay"][sl], rx, ry)
+        assert norm <= 1e-12, (i, norm)
+        off += n
+    eng.close()
+
+
+def test_interaction_counts():
+    d = inputs.gaussian_mixture(20000, 8, (0.01, 0.05), seed=14)
+    G = 40
+    eng = _forces_setup(d, G)
+    near, far = eng.interactions()[-1]
+    ref = orc.interactions(G, eng.counts().astype(np.int64))
+    assert (near, far) == ref
+    eng.close()

@@ -30,7 +30,7 @@ EXPORTED = (
     "bh_last_error", "bh_version", "bh_create", "bh_destroy", "bh_set_systems", "bh_set_tuning",
     "bh_get_tuning", "bh_set_cost_log", "bh_place_particles", "bh_calculate_box_cumm",
     "bh_sort_particles", "bh_summarize", "bh_build_items", "bh_evaluate_boxes",
-    "bh_tick_particles", "bh_step", "bh_step_graph", "bh_read_stats", "bh_export",
+    "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query",
 )
@@ -85,6 +85,7 @@ def load(path: str = LIB_PATH):
         "bh_summarize": (ctypes.c_int, [vp, P(BhParticles), vp]),
         "bh_build_items": (ctypes.c_int, [vp, vp]),
         "bh_evaluate_boxes": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), i32, vp, vp]),
+        "bh_force_pass": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), i32, vp, vp]),
         "bh_tick_particles": (ctypes.c_int, [vp, P(BhParticles), dbl, vp]),
         "bh_step": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), P(BhLaw), i32, dbl, i32,
                                    vp]),
@@ -222,6 +223,14 @@ class Context:
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         check(self.L.bh_evaluate_boxes(self.h, ctypes.byref(p), ctypes.byref(law), prec,
                                        ptr(order), stream_handle(stream)))
+
+    def force_pass(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", order=None,
+                   stream=None):
+        p = particles_struct(parts)
+        law = BhLaw(float(G), float(eps2), float(r2_cut))
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(self.L.bh_force_pass(self.h, ctypes.byref(p), ctypes.byref(law), prec, ptr(order),
+                                   stream_handle(stream)))
 
     def tick(self, parts, dt, stream=None):
         p = particles_struct(parts)

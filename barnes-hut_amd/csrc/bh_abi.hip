@@ -448,10 +448,24 @@ int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32
     if (precision < 0 || precision > 3) return fail(BH_EINVAL, "precision must be in [0, 3]");
     if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
         return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
-    hipStream_t st = (hipStream_t)stream;
     if (!order) {
         if ((r = bh_build_items(ctx, stream))) return r;
     }
+    return bh_force_pass(ctx, p, law, precision, order, stream);
+}
+
+int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
+                  const int32_t* order, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(p, "particles");
+    if (r) return r;
+    if (!law) return fail(BH_EINVAL, "law is NULL");
+    if (precision < 0 || precision > 3) return fail(BH_EINVAL, "precision must be in [0, 3]");
+    if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
+        return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
+    hipStream_t st = (hipStream_t)stream;
     const int pidx = c->tune.p == 1 ? 0 : (c->tune.p == 2 ? 1 : 2);
     const int lawi = law->r2_cut > 0.0 ? 1 : 0;
     const void* fn = bh_force_kernel(precision, pidx, lawi);

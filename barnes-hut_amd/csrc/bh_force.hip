@@ -11,8 +11,8 @@
 // and the result is added to the existing acceleration, as the reference
 // accumulates in place (gpu/grid.py:227-230).
 //
-// Work decomposition: a work item is a run of <= 64*P sorted particles of one
-// column cx spanning rows [y0, y1] (y1 - y0 < yspan).  One wave (a 64-thread
+// Work decomposition: a work item is a run of <= 64*P consecutive sorted
+// particles of one column cx spanning rows [y0, y1] (y1 - y0 < yspan).  One wave (a 64-thread
 // workgroup) takes items from a queue (atomic cursor; optional order array),
 // stages the item's neighbour particles and then the system's compact COM
 // list through LDS in 64-entry tiles, and every lane evaluates its P
@@ -24,11 +24,12 @@
 #include "bh_kernels.h"
 
 // ---------------------------------------------------------------- items
-// One thread per grid column (all systems).  Greedy packing of the column's
-// non-empty cells, in row order, into items of <= pitem particles and
-// < yspan rows; a cell with more than pitem particles is cut into pitem
-// chunks.  Pass 1 (WRITE = false) counts items per column; pass 2 writes
-// them at the scanned column offsets.
+// One thread per grid column (all systems).  The column's sorted particles
+// are cut, in order, into runs of pitem particles (a run may start or end
+// inside a cell); a run is closed early when its next particle lies yspan or
+// more rows below its first one, and at the end of the column.  Pass 1
+// (WRITE = false) counts items per column; pass 2 writes them at the scanned
+// column offsets.
 template <bool WRITE>
 __global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
                         const int32_t* __restrict__ counts, const int32_t* __restrict__ offsets,
@@ -46,17 +47,17 @@ __global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
     int n_out = 0;
     bool open = false;
     int p0 = 0, np = 0, y0 = 0, y1 = 0;
-    auto emit = [&](int ip0, int inp, int iy0, int iy1) {
+    auto emit = [&]() {
         if (WRITE) {
             int64_t k = out + n_out;
             if (k < max_items) {
                 Item it;
-                it.p0 = ip0;
-                it.np = inp;
+                it.p0 = p0;
+                it.np = np;
                 it.sys = s;
                 it.cx = x;
-                it.y0 = iy0;
-                it.y1 = iy1;
+                it.y0 = y0;
+                it.y1 = y1;
                 it.pad0 = 0;
                 it.pad1 = 0;
                 items[k] = it;
@@ -65,34 +66,29 @@ __global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
             }
         }
         ++n_out;
+        open = false;
     };
     for (int yy = 0; yy < G; ++yy) {
         int nc = counts[cb + yy];
         if (nc == 0) continue;
         int pc = offsets[cb + yy];
-        if (open && np + nc <= pitem && yy - y0 < yspan) {
-            np += nc;
+        if (open && yy - y0 >= yspan) emit();
+        while (nc > 0) {
+            if (!open) {
+                open = true;
+                p0 = pc;
+                np = 0;
+                y0 = yy;
+            }
+            int take = min(nc, pitem - np);
+            np += take;
+            pc += take;
+            nc -= take;
             y1 = yy;
-            continue;
-        }
-        if (open) {
-            emit(p0, np, y0, y1);
-            open = false;
-        }
-        while (nc > pitem) {
-            emit(pc, pitem, yy, yy);
-            pc += pitem;
-            nc -= pitem;
-        }
-        if (nc > 0) {
-            open = true;
-            p0 = pc;
-            np = nc;
-            y0 = yy;
-            y1 = yy;
+            if (np == pitem) emit();
         }
     }
-    if (open) emit(p0, np, y0, y1);
+    if (open) emit();
     if (!WRITE) col_items[col] = n_out;
 }
 

@@ -1,0 +1,294 @@
+#!/usr/bin/env python3
+"""Throughput of the grid-method step on one MI355X.
+
+Workload: N = 4,194,304 particles in 2-D, a mixture of 64 isotropic Gaussians
+(centres U[0,1)^2, sigma log-uniform [0.005, 0.05], clipped to the unit box),
+masses U[1,2], zero initial velocity, a 1024 x 1024 grid; law of the
+reference's GPU kernel (no softening, no cutoff), G = 1e-6, dt = 1e-3.
+
+A "step" is one full round: bounding box + placement + ordered permutation,
+per-cell COM, forces, tick.  Two paths run on the same input:
+  * plain FP64 (static schedule) -- the rebuilt reference path,
+  * FP32 cell-relative forces with Kahan-compensated tile sums and the
+    learned work-queue order when a trained cost model is given
+    (--cost-model), else the static order.
+`value` is the FP32 path's whole-job throughput in Mparticle-steps/s.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "barnes-hut_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "Mparticle-steps/s at N=4M 2D (FP32+learned sched vs plain); force rel err"
+N_DEFAULT = 4194304
+GRID_DEFAULT = 1024
+N_CLUSTERS = 64
+SIGMA = (0.005, 0.05)
+GC, DT, EPS2, R2CUT = 1e-6, 1e-3, 0.0, 0.0
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+FLOP_PER_INTERACTION = 11
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_path(eng, precision, steps, warmup, order_fn=None):
+    """Warm-up, then `steps` timed full steps.  Returns (seconds per step,
+    mean force-kernel ms, interactions of the last step)."""
+    import torch
+    ctx = eng.ctx
+
+    def one(events=None):
+        eng.create_tree()
+        eng.summarize()
+        ctx.build_items()
+        order = order_fn() if order_fn is not None else None
+        if events is not None:
+            events[0].record()
+        ctx.force_pass(eng.parts, GC, EPS2, R2CUT, precision, order)
+        if events is not None:
+            events[1].record()
+        eng.timestep(DT)
+
+    for _ in range(warmup):
+        one()
+    eng.check()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s in range(steps):
+        one(evs[s])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.check()
+    kms = [a.elapsed_time(b) for a, b in evs]
+    inter = eng.interactions()[-1]
+    return (t1 - t0) / steps, float(np.mean(kms)), (int(inter[0]), int(inter[1]))
+
+
+def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7):
+    """FP32 vs FP64 accelerations on the same placement: normwise, p99 and
+    max per-particle relative error over all N particles (FP64 GPU path as
+    the reference), plus FP64 GPU path vs the CPU oracle on a sample."""
+    import torch
+    eng = eng_cls([len(d["px"])], [G])
+    eng.load(d)
+    eng.create_tree()
+    eng.summarize()
+    eng.ctx.build_items()
+    eng.ctx.force_pass(eng.parts, GC, EPS2, R2CUT, "fp64")
+    a64 = (eng.parts["ax"].clone(), eng.parts["ay"].clone())
+    out = {}
+    for prec in ("fp32", "fp32_kahan_term", "fp32_plain"):
+        eng.parts["ax"].zero_()
+        eng.parts["ay"].zero_()
+        eng.ctx.force_pass(eng.parts, GC, EPS2, R2CUT, prec)
+        num = torch.hypot(eng.parts["ax"] - a64[0], eng.parts["ay"] - a64[1])
+        den = torch.hypot(a64[0], a64[1])
+        per = (num / den).cpu().numpy()
+        out[prec] = dict(normwise=float(torch.sqrt((num ** 2).sum() / (den ** 2).sum())),
+                         p99=float(np.quantile(per, 0.99)), max=float(per.max()))
+    eng.check()
+    # FP64 GPU vs the CPU oracle on sampled particles
+    from oracle import oracle as orc
+    h = eng.fetch()
+    off = eng.offsets().astype(np.int64)
+    keys = eng.keys()
+    com = orc.summarize(off, h["px"], h["py"], h["mass"])
+    rng = np.random.default_rng(seed)
+    tg = np.sort(rng.choice(len(h["px"]), oracle_sample, replace=False))
+    rx, ry = orc.forces(G, off, keys, h["px"], h["py"], h["mass"], com, GC, EPS2, R2CUT,
+                        targets=tg)
+    g64x, g64y = a64[0].cpu().numpy()[tg], a64[1].cpu().numpy()[tg]
+    per = np.hypot(g64x - rx, g64y - ry) / np.hypot(rx, ry)
+    out["fp64_vs_oracle_sample"] = dict(n=int(oracle_sample), max=float(per.max()))
+    eng.close()
+    return out
+
+
+def cpu_baseline(d, G, budget_s=12.0):
+    """The C restatement (oracle/) of python-bh's sequential step on this host:
+    full placement, COM and tick over all N; forces on a random sample of
+    target particles (OpenMP over targets), extrapolated to N."""
+    from oracle import oracle as orc
+    cores = len(os.sched_getaffinity(0))
+    n = len(d["px"])
+    z = np.zeros(n)
+    t0 = time.perf_counter()
+    bb = orc.bbox(d["px"], d["py"], G)
+    _, _, key = orc.place(d["px"], d["py"], bb[0], bb[1], bb[4], G)
+    perm = orc.sort_perm(key, (d["px"], d["py"], d["mass"], d["vx"], d["vy"], z, z,
+                               d["pid"].astype(np.float64)))
+    ks = key[perm]
+    x, y, m = d["px"][perm], d["py"][perm], d["mass"][perm]
+    _, off = orc.counts_offsets(ks, G * G)
+    t1 = time.perf_counter()
+    com = orc.summarize(off, x, y, m)
+    t2 = time.perf_counter()
+    rng = np.random.default_rng(11)
+    probe = np.sort(rng.choice(n, 256, replace=False))
+    ta = time.perf_counter()
+    orc.forces(G, off, ks, x, y, m, com, GC, EPS2, R2CUT, targets=probe, nthreads=cores)
+    per_target = (time.perf_counter() - ta) / len(probe)
+    ns = int(min(n, max(512, budget_s / max(per_target, 1e-9))))
+    tg = np.sort(rng.choice(n, ns, replace=False))
+    ta = time.perf_counter()
+    orc.forces(G, off, ks, x, y, m, com, GC, EPS2, R2CUT, targets=tg, nthreads=cores)
+    t_force = (time.perf_counter() - ta) * n / ns
+    vx, vy = d["vx"][perm].copy(), d["vy"][perm].copy()
+    ax, ay = np.zeros(n), np.zeros(n)
+    xx, yy = x.copy(), y.copy()
+    tb = time.perf_counter()
+    orc.tick(xx, yy, vx, vy, ax, ay, DT)
+    t_tick = time.perf_counter() - tb
+    t_step = (t1 - t0) + (t2 - t1) + t_force + t_tick
+    return {"value": n / t_step / 1e6, "unit": "Mparticle-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": (f"C restatement of the sequential step (oracle/bh_oracle.c): placement "
+                       f"({t1 - t0:.2f}s, 1 thread), COM ({t2 - t1:.2f}s) and tick over all "
+                       f"{n} particles; forces on {ns} random particles with {cores} OpenMP "
+                       f"threads, x{n / ns:.1f} extrapolated ({t_force:.1f}s/step)"),
+            "seconds_per_step": t_step}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=N_DEFAULT)
+    ap.add_argument("--grid", type=int, default=GRID_DEFAULT)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--p", type=int, default=2, help="FP32 path: particles per lane (1, 2, 4)")
+    ap.add_argument("--yspan", type=int, default=128, help="FP32 path: max rows per work item")
+    ap.add_argument("--waves-per-cu", type=int, default=16)
+    ap.add_argument("--p64", type=int, default=1, help="FP64 path: particles per lane")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick each path's launch shape with barneshut.autotune first")
+    ap.add_argument("--cost-model", default=None, help="trained cost-model file (.npz)")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-errors", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              f"the multi-GPU slab decomposition is a design, not a runnable path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from barneshut import inputs
+    from barneshut.engine import Engine
+
+    log(f"generating N={args.n} clusters={N_CLUSTERS} seed={args.seed}")
+    d = inputs.gaussian_mixture(args.n, N_CLUSTERS, SIGMA, seed=args.seed)
+    G = args.grid
+    tuning = dict(p=args.p, yspan=args.yspan, waves_per_cu=args.waves_per_cu)
+    tuning64 = dict(p=args.p64, yspan=args.yspan, waves_per_cu=args.waves_per_cu)
+
+    res64 = None
+    if not args.skip_fp64:
+        log("fp64 plain path")
+        e64 = Engine([args.n], [G], tuning=tuning64)
+        e64.load(d)
+        if args.autotune:
+            from barneshut import autotune
+            e64.create_tree()
+            e64.summarize()
+            tuning64 = autotune.tune(e64, (GC, EPS2, R2CUT), "fp64", log=log)["tuning"]
+            e64.load(d)
+        s64, k64, inter = run_path(e64, "fp64", args.steps, args.warmup)
+        res64 = dict(seconds_per_step=s64, value=args.n / s64 / 1e6, force_kernel_ms=k64)
+        log(f"fp64: {s64 * 1e3:.1f} ms/step, force kernel {k64:.1f} ms")
+        e64.close()
+        del e64
+
+    log("fp32 path")
+    e32 = Engine([args.n], [G], tuning=tuning)
+    e32.load(d)
+    if args.autotune:
+        from barneshut import autotune
+        e32.create_tree()
+        e32.summarize()
+        tuning = autotune.tune(e32, (GC, EPS2, R2CUT), "fp32", log=log)["tuning"]
+        e32.load(d)
+    order_fn = None
+    sched = "static"
+    if args.cost_model:
+        from barneshut import costmodel
+        model = costmodel.CostModel.load(args.cost_model, device=e32.dev)
+        order_buf = torch.empty(e32.ctx.query(4), dtype=torch.int32, device=e32.dev)
+
+        def order_fn():
+            e32.ctx.learned_order(model.params, order_buf)
+            return order_buf
+        sched = "learned"
+    s32, k32, inter = run_path(e32, "fp32", args.steps, args.warmup, order_fn)
+    log(f"fp32 ({sched}): {s32 * 1e3:.1f} ms/step, force kernel {k32:.1f} ms")
+    n_items = int(e32.ctx.stats()[2])
+    e32.close()
+    del e32
+
+    flops = FLOP_PER_INTERACTION * (inter[0] + inter[1])
+    achieved = flops / (k32 * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": args.n / s32 / 1e6,
+        "unit": "Mparticle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": s32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "2-D grid-method step, N=4,194,304, 64 Gaussian clusters, "
+                               "1024x1024 grid",
+                   "n": args.n, "grid": G, "clusters": N_CLUSTERS, "sigma": list(SIGMA),
+                   "mass": "U[1,2]", "G": GC, "dt": DT, "eps2": EPS2, "r2_cut": R2CUT,
+                   "seed": args.seed, "schedule": sched, "tuning": tuning,
+                   "work_items": n_items},
+        "interactions_per_step": {"near": inter[0], "far": inter[1]},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_force32<P,0,1>", "force_kernel_ms": k32,
+                     "flop_per_launch": flops},
+    }
+    if res64 is not None:
+        a64 = flops / (res64["force_kernel_ms"] * 1e-3) / 1e12
+        out["fp64_plain"] = {"value": res64["value"], "ms_per_step": res64["seconds_per_step"] * 1e3,
+                             "tuning": tuning64,
+                             "force_kernel_ms": res64["force_kernel_ms"],
+                             "achieved_tflops": a64, "frac_of_fp64_peak": a64 / FP64_PEAK_TFLOPS}
+        out["speedup_vs_fp64"] = res64["seconds_per_step"] / s32
+    if not args.skip_errors:
+        log("force errors")
+        out["force_rel_err"] = force_errors(d, G, Engine)
+    if not args.skip_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(d, G)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -109,6 +109,11 @@ int bh_summarize(void* ctx, const bh_particles* p, void* stream);
 int bh_build_items(void* ctx, void* stream);
 int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
                       const int32_t* order, void* stream);
+/* The force kernel alone, on the items of the last bh_build_items (order as
+ * in bh_evaluate_boxes; NULL = static order).  bh_evaluate_boxes with a NULL
+ * order is bh_build_items followed by this call. */
+int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
+                  const int32_t* order, void* stream);
 int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream);
 
 /* One full round: a is placed and sorted into b, then summarize, evaluate

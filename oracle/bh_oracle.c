@@ -199,10 +199,24 @@ void bho_forces(int G, const int64_t *offsets, const int32_t *key, int64_t n,
     /* compact list of non-empty cells in key order */
     int64_t K = 0;
     for (int64_t c = 0; c < ncells; ++c) K += (offsets[c + 1] > offsets[c]);
-    int32_t *ne = (int32_t *)malloc(sizeof(int32_t) * (size_t)(K > 0 ? K : 1));
+    size_t kb = (size_t)(K > 0 ? K : 1);
+    int32_t *ne = (int32_t *)malloc(sizeof(int32_t) * kb);
+    int32_t *nkx = (int32_t *)malloc(sizeof(int32_t) * kb);
+    int32_t *nky = (int32_t *)malloc(sizeof(int32_t) * kb);
+    double *nX = (double *)malloc(sizeof(double) * kb);
+    double *nY = (double *)malloc(sizeof(double) * kb);
+    double *nM = (double *)malloc(sizeof(double) * kb);
     K = 0;
     for (int64_t c = 0; c < ncells; ++c)
-        if (offsets[c + 1] > offsets[c]) ne[K++] = (int32_t)c;
+        if (offsets[c + 1] > offsets[c]) {
+            ne[K] = (int32_t)c;
+            nkx[K] = (int32_t)(c / G);
+            nky[K] = (int32_t)(c % G);
+            nX[K] = com[3 * c];
+            nY[K] = com[3 * c + 1];
+            nM[K] = com[3 * c + 2];
+            ++K;
+        }
     int64_t nt = targets ? ntargets : n;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -234,12 +248,10 @@ void bho_forces(int G, const int64_t *offsets, const int32_t *key, int64_t n,
         }
         /* far field */
         for (int64_t k = 0; k < K; ++k) {
-            int32_t cc = ne[k];
-            int kx = cc / G, ky = cc % G;
-            if (abs(kx - cx) <= 1 && abs(ky - cy) <= 1) continue;
-            double wx = com[3 * (int64_t)cc] - xi, wy = com[3 * (int64_t)cc + 1] - yi;
+            if (abs(nkx[k] - cx) <= 1 && abs(nky[k] - cy) <= 1) continue;
+            double wx = nX[k] - xi, wy = nY[k] - yi;
             double s = wx * wx + wy * wy;
-            double w = law_w(s, eps2, r2_cut) * com[3 * (int64_t)cc + 2];
+            double w = law_w(s, eps2, r2_cut) * nM[k];
             sx += wx * w;
             sy += wy * w;
         }
@@ -247,6 +259,11 @@ void bho_forces(int G, const int64_t *offsets, const int32_t *key, int64_t n,
         ay[t] = Gc * sy;
     }
     free(ne);
+    free(nkx);
+    free(nky);
+    free(nX);
+    free(nY);
+    free(nM);
 }
 
 /* O(S*N) direct sum, implementations/base.py:16-32 (nsquared_acc_check):

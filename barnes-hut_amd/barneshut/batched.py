@@ -1,0 +1,67 @@
+"""Many independent systems stepped together on one GPU.
+
+The reference runs K independent BarnesHut objects in Python threads
+(python-bh/bin/run_2d_concurrent.py:29-90).  Here all systems share one
+context: their particles are concatenated, every kernel covers all systems
+(cells of system s live at global indices [c_off_s, c_off_s + G_s^2)), the
+force kernel drains one work queue that holds the items of every system, and
+a whole step is replayed as one captured HIP graph.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .engine import Engine
+
+
+class BatchedSystems:
+
+    def __init__(self, sizes, grid_dims, data, law=(1.0, 0.0, 0.0), precision="fp32", dt=0.0,
+                 tuning=None, device=0, model=None, graph=True):
+        self.sizes = np.asarray(sizes, np.int64)
+        self.grid_dims = np.asarray(grid_dims, np.int32)
+        self.eng = Engine(self.sizes, self.grid_dims, device=device, tuning=tuning)
+        self.eng.load(data)
+        self.law = law
+        self.precision = precision
+        self.dt = dt
+        self.graph = graph
+        self.model = model
+        self.order = None
+        if model is not None:
+            import torch
+            self.order = torch.empty(self.eng.ctx.query(4), dtype=torch.int32, device=self.eng.dev)
+
+    def step(self):
+        e = self.eng
+        G, eps2, r2 = self.law
+        if self.model is None:
+            e.ctx.step(e.parts, e.other, G, eps2, r2, self.precision, self.dt, True, self.graph)
+        else:
+            e.ctx.step_learned(e.parts, e.other, self.model.params, self.order, G, eps2, r2,
+                               self.precision, self.dt, True, self.graph)
+        e.cur = 1 - e.cur
+
+    def run(self, steps, warmup=1):
+        """Returns seconds per step (synchronised wall clock)."""
+        import torch
+        for _ in range(warmup):
+            self.step()
+        self.eng.check()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            self.step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        self.eng.check()
+        return dt
+
+    def system_slices(self):
+        off = np.concatenate([[0], np.cumsum(self.sizes)])
+        return [slice(int(off[i]), int(off[i + 1])) for i in range(len(self.sizes))]
+
+    def close(self):
+        self.eng.close()

@@ -30,7 +30,7 @@ EXPORTED = (
     "bh_last_error", "bh_version", "bh_create", "bh_destroy", "bh_set_systems", "bh_set_tuning",
     "bh_get_tuning", "bh_set_cost_log", "bh_place_particles", "bh_calculate_box_cumm",
     "bh_sort_particles", "bh_summarize", "bh_build_items", "bh_evaluate_boxes",
-    "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_read_stats", "bh_export",
+    "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query",
 )
@@ -91,6 +91,8 @@ def load(path: str = LIB_PATH):
                                    vp]),
         "bh_step_graph": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), P(BhLaw), i32, dbl,
                                          i32, vp]),
+        "bh_step_learned": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), P(BhLaw), i32,
+                                           dbl, i32, vp, vp, i32, vp]),
         "bh_read_stats": (ctypes.c_int, [vp, P(i32), vp]),
         "bh_export": (ctypes.c_int, [vp, i32, vp, i64, vp]),
         "bh_interactions": (ctypes.c_int, [vp, P(i64), vp]),
@@ -244,6 +246,15 @@ class Context:
         fn = self.L.bh_step_graph if graph else self.L.bh_step
         check(fn(self.h, ctypes.byref(pa), ctypes.byref(pb), ctypes.byref(law), prec, float(dt),
                  1 if do_tick else 0, stream_handle(stream)))
+
+    def step_learned(self, a, b, params, order, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp32",
+                     dt=0.0, do_tick=True, graph=False, stream=None):
+        pa, pb = particles_struct(a), particles_struct(b)
+        law = BhLaw(float(G), float(eps2), float(r2_cut))
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        check(self.L.bh_step_learned(self.h, ctypes.byref(pa), ctypes.byref(pb), ctypes.byref(law),
+                                     prec, float(dt), 1 if do_tick else 0, ptr(params), ptr(order),
+                                     1 if graph else 0, stream_handle(stream)))
 
     # ---- state
     def stats(self, stream=None, raise_on_error=True):

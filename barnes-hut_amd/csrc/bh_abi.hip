@@ -43,6 +43,8 @@ struct GraphKey {
     double dt;
     int do_tick;
     bh_tuning tune;
+    const float* params;
+    int32_t* order;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -510,21 +512,37 @@ int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream)
     return BH_OK;
 }
 
-int bh_step(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
-            int32_t precision, double dt, int32_t do_tick, void* stream)
+extern "C" int bh_learned_order(void* ctx, const float* params, int32_t* order, void* stream);
+
+static int step_impl(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+                     int32_t precision, double dt, int32_t do_tick, const float* params,
+                     int32_t* order, void* stream)
 {
     int r;
     if ((r = bh_place_particles(ctx, a, stream))) return r;
     if ((r = bh_calculate_box_cumm(ctx, stream))) return r;
     if ((r = bh_sort_particles(ctx, a, b, stream))) return r;
     if ((r = bh_summarize(ctx, b, stream))) return r;
-    if ((r = bh_evaluate_boxes(ctx, b, law, precision, nullptr, stream))) return r;
+    if ((r = bh_build_items(ctx, stream))) return r;
+    const int32_t* ord = nullptr;
+    if (params) {
+        if ((r = bh_learned_order(ctx, params, order, stream))) return r;
+        ord = order;
+    }
+    if ((r = bh_force_pass(ctx, b, law, precision, ord, stream))) return r;
     if (do_tick && (r = bh_tick_particles(ctx, b, dt, stream))) return r;
     return BH_OK;
 }
 
-int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
-                  int32_t precision, double dt, int32_t do_tick, void* stream)
+int bh_step(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+            int32_t precision, double dt, int32_t do_tick, void* stream)
+{
+    return step_impl(ctx, a, b, law, precision, dt, do_tick, nullptr, nullptr, stream);
+}
+
+static int step_graph_impl(void* ctx, const bh_particles* a, const bh_particles* b,
+                           const bh_law* law, int32_t precision, double dt, int32_t do_tick,
+                           const float* params, int32_t* order, void* stream)
 {
     Ctx* c = (Ctx*)ctx;
     CTX_READY(c);
@@ -541,6 +559,8 @@ int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const
     k.dt = dt;
     k.do_tick = do_tick;
     k.tune = c->tune;
+    k.params = params;
+    k.order = order;
     int slot = -1;
     for (int i = 0; i < 2; ++i)
         if (c->gexec[i] && c->gkey[i] == k) slot = i;
@@ -552,7 +572,7 @@ int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const
             c->gexec[slot] = nullptr;
         }
         HIP_TRY(hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal));
-        int rr = bh_step(ctx, a, b, law, precision, dt, do_tick, c->cap_stream);
+        int rr = step_impl(ctx, a, b, law, precision, dt, do_tick, params, order, c->cap_stream);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(c->cap_stream, &g);
         if (rr) {
@@ -572,6 +592,21 @@ int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const
     HIP_TRY(hipEventRecord(c->ev_out, c->cap_stream));
     HIP_TRY(hipStreamWaitEvent(st, c->ev_out, 0));
     return BH_OK;
+}
+
+int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+                  int32_t precision, double dt, int32_t do_tick, void* stream)
+{
+    return step_graph_impl(ctx, a, b, law, precision, dt, do_tick, nullptr, nullptr, stream);
+}
+
+int bh_step_learned(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+                    int32_t precision, double dt, int32_t do_tick, const float* params,
+                    int32_t* order, int32_t graph, void* stream)
+{
+    if (!params || !order) return fail(BH_EINVAL, "params and order are required");
+    if (graph) return step_graph_impl(ctx, a, b, law, precision, dt, do_tick, params, order, stream);
+    return step_impl(ctx, a, b, law, precision, dt, do_tick, params, order, stream);
 }
 
 int bh_read_stats(void* ctx, int32_t* out16, void* stream)

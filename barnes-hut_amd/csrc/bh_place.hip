@@ -413,29 +413,32 @@ __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ key
 // ---------------------------------------------------------------- within-cell order
 // numpy's structured-sort tie rule after (gx, gy): px, py, mass, vx, vy, ax,
 // ay, pid (internals/particle.py:3-9); equal rows keep input order (stable).
+// Doubles compare as numpy sorts them: NaN after every number, NaNs equal,
+// so the order is total and the ranks always form a permutation.
+__device__ __forceinline__ int dcmp(double u, double v)
+{
+    const bool un = isnan(u), vn = isnan(v);
+    if (un || vn) return (int)un - (int)vn;
+    return (u < v) ? -1 : (u > v ? 1 : 0);
+}
+
 __device__ __forceinline__ bool tie_less(const SortFields& F, int a, int b)
 {
-    double u, v;
-    u = F.py[a]; v = F.py[b];
-    if (u != v) return u < v;
-    u = F.m[a]; v = F.m[b];
-    if (u != v) return u < v;
-    u = F.vx[a]; v = F.vx[b];
-    if (u != v) return u < v;
-    u = F.vy[a]; v = F.vy[b];
-    if (u != v) return u < v;
-    u = F.ax[a]; v = F.ax[b];
-    if (u != v) return u < v;
-    u = F.ay[a]; v = F.ay[b];
-    if (u != v) return u < v;
+    int c;
+    if ((c = dcmp(F.py[a], F.py[b]))) return c < 0;
+    if ((c = dcmp(F.m[a], F.m[b]))) return c < 0;
+    if ((c = dcmp(F.vx[a], F.vx[b]))) return c < 0;
+    if ((c = dcmp(F.vy[a], F.vy[b]))) return c < 0;
+    if ((c = dcmp(F.ax[a], F.ax[b]))) return c < 0;
+    if ((c = dcmp(F.ay[a], F.ay[b]))) return c < 0;
     if (F.pid[a] != F.pid[b]) return F.pid[a] < F.pid[b];
     return a < b;
 }
 
 __device__ __forceinline__ bool row_less(const SortFields& F, int a, int b)
 {
-    double u = F.px[a], v = F.px[b];
-    if (u != v) return u < v;
+    int c = dcmp(F.px[a], F.px[b]);
+    if (c) return c < 0;
     return tie_less(F, a, b);
 }
 
@@ -482,8 +485,8 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
         int rank = 0;
         for (int j = (int)(ua - w0); j < (int)(ub - w0); ++j) {
             if (j == t) continue;
-            double xj = lpx[j];
-            rank += (xj < x) || (xj == x && tie_less(F, lidx[j], e));
+            const int c = dcmp(lpx[j], x);
+            rank += (c < 0) || (c == 0 && tie_less(F, lidx[j], e));
         }
         idx_out[ua + rank] = e;
     }
@@ -551,6 +554,7 @@ __global__ __launch_bounds__(256) void k_gather(SortFields S, BhParticles D, int
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += stride) {
         int k = key_sorted[q];
         int e = (offsets[k + 1] - offsets[k] <= SORT_W) ? idx_b[q] : big[q];
+        if ((uint32_t)e >= (uint32_t)N) e = 0;   // never taken: the ranks form a permutation
         D.px[q] = S.px[e];
         D.py[q] = S.py[e];
         D.m[q] = S.m[e];

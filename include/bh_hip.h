@@ -125,6 +125,14 @@ int bh_step(void* ctx, const bh_particles* a, const bh_particles* b, const bh_la
 int bh_step_graph(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
                   int32_t precision, double dt, int32_t do_tick, void* stream);
 
+/* bh_step with the learned queue order: after the items are built,
+ * bh_learned_order(params -> order) runs and the force pass follows that
+ * order.  order: device int32 of at least the work-item capacity
+ * (bh_query 4).  graph != 0 replays it from a captured HIP graph. */
+int bh_step_learned(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
+                    int32_t precision, double dt, int32_t do_tick, const float* params,
+                    int32_t* order, int32_t graph, void* stream);
+
 /* Synchronises the stream, copies the int32 stats block (16 entries:
  * [0] error bits, [1] max cell count, [2] work items, [4] non-empty cells)
  * to host memory out16, and returns BH_EDEVICE if an error bit is set. */

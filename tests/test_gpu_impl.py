@@ -1,0 +1,153 @@
+"""The drop-in: python-bh's BarnesHut(ini) / run() / run_2d.py interface
+driving the HIP implementation, the --check harness, pid-ordered results,
+and the device error path."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+from oracle import oracle as orc  # noqa: E402
+from barneshut import inputs  # noqa: E402
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _ini(tmp_path, impl="singlegpu", precision="fp64", grid_dim=0, eps2=0.0, rounds=1,
+         skip=False, G=1.0, tick=0.005, extra=""):
+    f = tmp_path / "t.ini"
+    f.write_text(f"""[general]
+implementation = {impl}
+sample_check_size = 10
+skip_timestep = {skip}
+evaluation_rounds = 1
+rounds = {rounds}
+[grid]
+max_particles_per_box = 1000
+[bh]
+grav_constant = {G}
+tick_seconds = {tick}
+[hip]
+precision = {precision}
+eps2 = {eps2}
+r2_cut = 0
+grid_dim = {grid_dim}
+{extra}
+""")
+    return str(f)
+
+
+def _input(tmp_path, n=3000, seed=1):
+    d = inputs.reference_normal(n, seed)
+    f = tmp_path / "in.txt"
+    inputs.write_particles_file(str(f), d)
+    return str(f), d
+
+
+def test_run_2d_cli_with_check_equals_direct_sum_on_2x2_grid(tmp_path):
+    """grid_dim = 2: every cell is a neighbour, so --check compares the
+    implementation with the exact direct sum."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "run_2d", os.path.join(ROOT, "barnes-hut_amd", "bin", "run_2d.py"))
+    run_2d = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(run_2d)
+    fin, d = _input(tmp_path)
+    ini = _ini(tmp_path, grid_dim=2, eps2=1e-6)
+    bh = run_2d.main([fin, "1", "2", ini, "--check"])
+    chk = bh.bh.last_check
+    assert max(chk["vector_rel"]) < 1e-10
+    bh.bh.close()
+
+
+def test_rounds_track_oracle_steps(tmp_path):
+    fin, d = _input(tmp_path, 4000, seed=2)
+    G = orc.grid_dim(4000, 1000)
+    ini = _ini(tmp_path, rounds=3, eps2=1e-6)
+    from barneshut import BarnesHut
+    bh = BarnesHut(ini)
+    bh.read_input_file(fin)
+    bh.run()
+    got = bh.bh.get_particles()
+    st = dict(d)
+    st["ax"] = np.zeros(4000)
+    st["ay"] = np.zeros(4000)
+    for _ in range(3):
+        o = orc.step(st["px"], st["py"], st["mass"], st["vx"], st["vy"], st["pid"], G, 1.0, 0.005,
+                     1e-6, 0.0, st["ax"], st["ay"])
+        inv = np.argsort(o["pid"])
+        st = dict(px=o["px_next"][inv], py=o["py_next"][inv], mass=o["m"][inv],
+                  vx=o["vx_next"][inv], vy=o["vy_next"][inv], pid=o["pid"][inv],
+                  ax=np.zeros(4000), ay=np.zeros(4000))
+    np.testing.assert_array_equal(got[:, 9], np.arange(4000))
+    np.testing.assert_allclose(got[:, 0], st["px"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(got[:, 3], st["vx"], rtol=1e-7, atol=1e-9)
+    bh.bh.close()
+
+
+def test_grid_columns_and_pid_order(tmp_path):
+    fin, d = _input(tmp_path, 2000, seed=3)
+    ini = _ini(tmp_path, skip=True)
+    from barneshut import BarnesHut
+    bh = BarnesHut(ini)
+    bh.read_input_file(fin)
+    bh.run()
+    a = bh.bh.get_particles()
+    G = bh.bh.grid_dim_
+    bb = orc.bbox(d["px"], d["py"], G)
+    gx, gy, _ = orc.place(d["px"], d["py"], bb[0], bb[1], bb[4], G)
+    np.testing.assert_array_equal(a[:, 7], gx)
+    np.testing.assert_array_equal(a[:, 8], gy)
+    np.testing.assert_array_equal(a[:, 0], d["px"])      # skip_timestep: positions unchanged
+    assert np.abs(a[:, 5]).max() > 0                      # accelerations kept
+    bh.bh.close()
+
+
+def test_fp32_precision_through_ini(tmp_path):
+    fin, d = _input(tmp_path, 20000, seed=4)
+    from barneshut import BarnesHut
+    r = {}
+    for prec in ("fp64", "fp32"):
+        bh = BarnesHut(_ini(tmp_path, precision=prec, skip=True, grid_dim=64))
+        bh.read_input_file(fin)
+        bh.run()
+        r[prec] = bh.bh.get_particles()[:, 5:7].copy()
+        bh.bh.close()
+    num = np.linalg.norm(r["fp32"] - r["fp64"])
+    assert num / np.linalg.norm(r["fp64"]) < 1e-5
+
+
+def test_nonfinite_coordinate_raises(tmp_path):
+    """A NaN coordinate is flagged by the bounding-box kernel; every kernel
+    still completes (keys are clamped, the row order stays total) and the
+    implementation raises."""
+    d = inputs.uniform(3000, seed=5)
+    d["px"][17] = np.nan
+    f = tmp_path / "nan.txt"
+    inputs.write_particles_file(str(f), d)
+    from barneshut import BarnesHut
+    from barneshut.kernels.hiplib import BhError
+    bh = BarnesHut(_ini(tmp_path, grid_dim=16))
+    bh.read_input_file(str(f))
+    with pytest.raises(BhError, match="non-finite"):
+        bh.run()
+    bh.bh.close()
+
+
+def test_invalid_arguments_raise():
+    from barneshut.engine import Engine
+    from barneshut.kernels.hiplib import BhError
+    with pytest.raises(BhError):
+        Engine([100], [0])
+    e = Engine([1000], [8])
+    with pytest.raises(BhError):
+        e.ctx.set_tuning(p=3)
+    with pytest.raises(BhError):
+        e.ctx.evaluate(e.parts, 1.0, -1.0, 0.0, "fp64")
+    with pytest.raises(BhError):
+        e.ctx.sort(e.parts, e.parts)
+    e.close()

@@ -1,0 +1,124 @@
+"""Host-side logic that needs no GPU: configuration, grid sizing, input
+format, generators, the --check direct sum, cost-model parameter folding,
+the implementation factory, the Timer, and bench.py's --gpus guard."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mlp as omlp
+from oracle import oracle as orc
+from barneshut import costmodel, helpers, inputs
+from barneshut.implementations.base import nsquared_acc_check
+from barneshut.internals.config import Config
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CFG = os.path.join(ROOT, "barnes-hut_amd", "configs")
+
+
+@pytest.mark.parametrize("n,ppb", [(16384, 23), (20000, 1000), (10_000_000, 1000), (77, 1),
+                                   (4194304, 6)])
+def test_grid_dim_for_matches_oracle(n, ppb):
+    assert helpers.grid_dim_for(n, ppb) == orc.grid_dim(n, ppb)
+
+
+def test_neighbor_cells():
+    assert sorted(helpers.get_neighbor_cells((0, 0), 4)) == [(0, 1), (1, 0), (1, 1)]
+    assert len(helpers.get_neighbor_cells((2, 2), 5)) == 8
+
+
+def test_config_reads_keys(tmp_path):
+    Config.read_file(os.path.join(CFG, "singlegpu.ini"))
+    assert Config.get("general", "implementation") == "singlegpu"
+    assert int(Config.get("grid", "max_particles_per_box")) == 1000
+    assert Config.get("hip", "precision") == "fp64"
+    with pytest.raises(FileNotFoundError):
+        Config.read_file(str(tmp_path / "missing.ini"))
+
+
+def test_factory_rejects_unprovided_implementation(tmp_path):
+    ini = tmp_path / "seq.ini"
+    ini.write_text("[general]\nimplementation = sequential\n[grid]\nmax_particles_per_box = 10\n")
+    from barneshut import BarnesHut
+    with pytest.raises(NotImplementedError):
+        BarnesHut(str(ini))
+
+
+def test_particle_file_roundtrip(tmp_path):
+    d = inputs.reference_normal(1234, seed=3)
+    f = tmp_path / "in.txt"
+    inputs.write_particles_file(str(f), d)
+    assert os.path.getsize(f) == 4 + 40 * 1234
+    e = inputs.read_particles_file(str(f))
+    for k in ("px", "py", "mass", "vx", "vy", "pid"):
+        np.testing.assert_array_equal(e[k], d[k])
+
+
+def test_generators():
+    d = inputs.gaussian_mixture(20000, 64, (0.005, 0.05), seed=1)
+    assert d["px"].min() >= 0.0 and d["px"].max() < 1.0
+    assert d["mass"].min() >= 1.0 and d["mass"].max() <= 2.0
+    d2 = inputs.gaussian_mixture(20000, 64, (0.005, 0.05), seed=1)
+    np.testing.assert_array_equal(d["px"], d2["px"])
+    d3 = inputs.gaussian_mixture(10000, 16, (0.001, 0.03), seed=2, mass="lognormal",
+                                 background=0.2)
+    assert len(d3["px"]) == 10000 and (d3["mass"] > 0).all()
+    r = inputs.reference_normal(5000, seed=4)
+    assert r["px"].max() < 499 and r["mass"].min() >= 0.5 and np.abs(r["vx"]).max() <= 1
+    n, g = inputs.batched_sizes(512, 1024, 262144, seed=0)
+    assert n.min() >= 1024 and n.max() <= 262144 and len(n) == 512
+    np.testing.assert_array_equal(g, np.ceil(np.sqrt(n / 16.0)).astype(np.int32))
+
+
+def test_nsquared_check_matches_oracle_direct():
+    d = inputs.uniform(500, seed=5)
+    a = np.zeros((500, 10))
+    a[:, 0], a[:, 1], a[:, 2], a[:, 9] = d["px"], d["py"], d["mass"], d["pid"]
+    idx = [3, 77, 401]
+    s = a[idx].copy()
+    nsquared_acc_check(a, s, 1.5, 1e-6, 0.0)
+    rx, ry = orc.direct(d["px"], d["py"], d["mass"], 1.5, 1e-6, 0.0, targets=np.array(idx))
+    np.testing.assert_allclose(s[:, 5], rx, rtol=1e-11)
+    np.testing.assert_allclose(s[:, 6], ry, rtol=1e-11)
+
+
+def test_costmodel_folding_equals_standardised_forward():
+    rng = np.random.default_rng(7)
+    p = costmodel.init_params(3)
+    X = rng.normal(3.0, 2.0, size=(50, 8))
+    mu, sd = X.mean(0), X.std(0)
+    folded = costmodel.fold_standardisation(p, mu, sd)
+    np.testing.assert_allclose(omlp.forward(folded, X), omlp.forward(p, (X - mu) / sd),
+                               rtol=1e-5, atol=1e-5)
+
+
+def test_costmodel_layout_matches_oracle():
+    assert costmodel.NPARAM == omlp.NPARAM == 1377
+    assert (costmodel.OFF_W2, costmodel.OFF_B3) == (omlp.OFF_W2, omlp.OFF_B3)
+
+
+def test_rank_quality():
+    assert costmodel.rank_quality([1, 2, 3, 4], [10, 20, 30, 40]) == pytest.approx(1.0)
+
+
+def test_timer_interface(capsys):
+    from timer import Timer
+    Timer.reset()
+    for _ in range(3):
+        with Timer.get_handle("evaluation"):
+            pass
+    assert len(Timer.times["evaluation"]) == 3
+    Timer.print()
+    out = capsys.readouterr().out
+    assert "starting evaluation" in out and "median" in out
+    Timer.reset()
+    assert Timer.times == {}
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr

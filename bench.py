@@ -187,6 +187,9 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-errors", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -198,6 +201,13 @@ def main():
     from barneshut import inputs
     from barneshut.engine import Engine
 
+    if args.calibrate_copy_mib:
+        src = torch.ones(args.calibrate_copy_mib * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB read + written")
+        del src, dst
     log(f"generating N={args.n} clusters={N_CLUSTERS} seed={args.seed}")
     d = inputs.gaussian_mixture(args.n, N_CLUSTERS, SIGMA, seed=args.seed)
     G = args.grid

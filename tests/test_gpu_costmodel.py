@@ -149,7 +149,7 @@ def test_collect_train_rank_on_logged_costs():
     d = inputs.gaussian_mixture(200000, 32, (0.002, 0.05), seed=9, mass="lognormal")
     e = Engine([len(d["px"])], [256], tuning=dict(p=2, yspan=32, waves_per_cu=8))
     e.load(d)
-    X, c = costmodel.collect(e, 4, (1e-6, 0.0, 0.0), "fp32", dt=1e-3)
+    X, c = costmodel.collect(e, 4, (1e-6, 1e-10, 0.0), "fp32", dt=1e-3)
     assert len(X) == len(c) and (c > 0).all()
     model, hist = costmodel.train(X, c, epochs=30, batch=8192, lr=3e-3)
     assert hist[-1][1] < hist[0][1]

@@ -31,7 +31,7 @@ EXPORTED = (
     "bh_get_tuning", "bh_set_cost_log", "bh_place_particles", "bh_calculate_box_cumm",
     "bh_sort_particles", "bh_summarize", "bh_build_items", "bh_evaluate_boxes",
     "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
-    "bh_interactions", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
+    "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query",
 )
 
@@ -96,6 +96,7 @@ def load(path: str = LIB_PATH):
         "bh_read_stats": (ctypes.c_int, [vp, P(i32), vp]),
         "bh_export": (ctypes.c_int, [vp, i32, vp, i64, vp]),
         "bh_interactions": (ctypes.c_int, [vp, P(i64), vp]),
+        "bh_interactions_async": (ctypes.c_int, [vp, vp, vp]),
         "bh_item_features": (ctypes.c_int, [vp, vp, vp]),
         "bh_mlp_forward": (ctypes.c_int, [vp, vp, i64, vp, vp]),
         "bh_mlp_backward": (ctypes.c_int, [vp, vp, vp, i64, vp, vp, vp]),
@@ -283,6 +284,10 @@ class Context:
         check(self.L.bh_interactions(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                      stream_handle(stream)))
         return out
+
+    def interactions_async(self, dev_out, stream=None):
+        """dev_out: device int64 tensor [2*S+2] (near counts include self pairs)."""
+        check(self.L.bh_interactions_async(self.h, ptr(dev_out), stream_handle(stream)))
 
     # ---- cost model
     def item_features(self, feat, stream=None):

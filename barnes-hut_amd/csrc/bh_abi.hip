@@ -686,6 +686,21 @@ int bh_query(void* ctx, int32_t what, int64_t* out)
     return BH_OK;
 }
 
+int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!dev_out) return fail(BH_EINVAL, "dev_out is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n2 = 2 * (int64_t)c->S + 2;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, (int32_t*)dev_out, 2 * n2, 0);
+    hipLaunchKernelGGL(k_interactions, dim3(grid_for(c->C, 256, 8192)), dim3(256), 0, st,
+                       (const SysDev*)c->sys, c->S, (const int32_t*)c->counts,
+                       (const int32_t*)c->neprefix, c->C, (unsigned long long*)dev_out);
+    LAUNCH_CHECK("interactions");
+    return BH_OK;
+}
+
 int bh_interactions(void* ctx, int64_t* out, void* stream)
 {
     Ctx* c = (Ctx*)ctx;

@@ -42,6 +42,11 @@ GC, DT, EPS2, R2CUT = 1e-6, 1e-3, 0.0, 0.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_INTERACTION = 11
+# HBM bytes per force-kernel launch on this workload: rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled by the readings of a
+# 1 GiB device copy (FETCH x2.0, WRITE x1.0), measured on the first two
+# steps of this workload with the default launch shape.
+TRAFFIC = 2.02e9
 
 
 def log(msg):
@@ -54,8 +59,13 @@ def run_path(eng, precision, steps, warmup, order_fn=None):
     import torch
     ctx = eng.ctx
 
-    def one(events=None):
+    n_tot = int(eng.n.sum())
+    counts = torch.zeros((steps, 4), dtype=torch.int64, device=eng.dev)
+
+    def one(events=None, slot=None):
         eng.create_tree()
+        if slot is not None:
+            ctx.interactions_async(counts[slot])
         eng.summarize()
         ctx.build_items()
         order = order_fn() if order_fn is not None else None
@@ -74,13 +84,15 @@ def run_path(eng, precision, steps, warmup, order_fn=None):
            for _ in range(steps)]
     t0 = time.perf_counter()
     for s in range(steps):
-        one(evs[s])
+        one(evs[s], s)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     eng.check()
-    kms = [a.elapsed_time(b) for a, b in evs]
-    inter = eng.interactions()[-1]
-    return (t1 - t0) / steps, float(np.mean(kms)), (int(inter[0]), int(inter[1]))
+    kms = np.array([a.elapsed_time(b) for a, b in evs])
+    c = counts.cpu().numpy()
+    near = c[:, 2] - n_tot
+    far = c[:, 3]
+    return (t1 - t0) / steps, kms, near, far
 
 
 def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7):
@@ -225,9 +237,12 @@ def main():
             e64.summarize()
             tuning64 = autotune.tune(e64, (GC, EPS2, R2CUT), "fp64", log=log)["tuning"]
             e64.load(d)
-        s64, k64, inter = run_path(e64, "fp64", args.steps, args.warmup)
-        res64 = dict(seconds_per_step=s64, value=args.n / s64 / 1e6, force_kernel_ms=k64)
-        log(f"fp64: {s64 * 1e3:.1f} ms/step, force kernel {k64:.1f} ms")
+        s64, k64, near64, far64 = run_path(e64, "fp64", args.steps, args.warmup)
+        f64 = FLOP_PER_INTERACTION * (near64 + far64).astype(np.float64)
+        res64 = dict(seconds_per_step=s64, value=args.n / s64 / 1e6,
+                     force_kernel_ms=float(k64.mean()),
+                     achieved=float(f64.sum() / (k64.sum() * 1e-3) / 1e12))
+        log(f"fp64: {s64 * 1e3:.1f} ms/step, force kernel {k64.mean():.1f} ms")
         e64.close()
         del e64
 
@@ -251,14 +266,16 @@ def main():
             e32.ctx.learned_order(model.params, order_buf)
             return order_buf
         sched = "learned"
-    s32, k32, inter = run_path(e32, "fp32", args.steps, args.warmup, order_fn)
+    s32, k32v, near32, far32 = run_path(e32, "fp32", args.steps, args.warmup, order_fn)
+    k32 = float(k32v.mean())
     log(f"fp32 ({sched}): {s32 * 1e3:.1f} ms/step, force kernel {k32:.1f} ms")
     n_items = int(e32.ctx.stats()[2])
     e32.close()
     del e32
 
-    flops = FLOP_PER_INTERACTION * (inter[0] + inter[1])
-    achieved = flops / (k32 * 1e-3) / 1e12
+    f32 = FLOP_PER_INTERACTION * (near32 + far32).astype(np.float64)
+    flops = float(f32.mean())
+    achieved = float(f32.sum() / (k32v.sum() * 1e-3) / 1e12)
     out = {
         "metric": METRIC,
         "value": args.n / s32 / 1e6,
@@ -278,14 +295,14 @@ def main():
                    "mass": "U[1,2]", "G": GC, "dt": DT, "eps2": EPS2, "r2_cut": R2CUT,
                    "seed": args.seed, "schedule": sched, "tuning": tuning,
                    "work_items": n_items},
-        "interactions_per_step": {"near": inter[0], "far": inter[1]},
+        "interactions_per_step": {"near": float(near32.mean()), "far": float(far32.mean())},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": TRAFFIC,
                      "kernel": "k_force32<P,0,1>", "force_kernel_ms": k32,
                      "flop_per_launch": flops},
     }
     if res64 is not None:
-        a64 = flops / (res64["force_kernel_ms"] * 1e-3) / 1e12
+        a64 = res64["achieved"]
         out["fp64_plain"] = {"value": res64["value"], "ms_per_step": res64["seconds_per_step"] * 1e3,
                              "tuning": tuning64,
                              "force_kernel_ms": res64["force_kernel_ms"],

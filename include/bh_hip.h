@@ -142,6 +142,10 @@ int bh_export(void* ctx, int32_t what, void* dst, int64_t count, void* stream);
 /* Exact interaction counts of the current placement: host out[2*S+2] =
  * (near_s, far_s) per system, then the totals. Synchronises. */
 int bh_interactions(void* ctx, int64_t* out, void* stream);
+/* Same counts without a host sync: device uint64 dev_out[2*S+2] receives
+ * sum_c n_c * (particles in the 3x3 block) (self pairs included: subtract
+ * each system's N for the near count) and the far counts. */
+int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream);
 
 /* Learned cost model (MLP 8 -> 32 -> 32 -> 1, FP32, ReLU).
  * params: device float[BH_MLP_NPARAM] = W1[32x8], b1[32], W2[32x32], b2[32],

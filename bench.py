@@ -7,12 +7,17 @@ masses U[1,2], zero initial velocity, a 1024 x 1024 grid; law of the
 reference's GPU kernel (no softening, no cutoff), G = 1e-6, dt = 1e-3.
 
 A "step" is one full round: bounding box + placement + ordered permutation,
-per-cell COM, forces, tick.  Two paths run on the same input:
-  * plain FP64 (static schedule) -- the rebuilt reference path,
-  * FP32 cell-relative forces with Kahan-compensated tile sums and the
-    learned work-queue order when a trained cost model is given
-    (--cost-model), else the static order.
-`value` is the FP32 path's whole-job throughput in Mparticle-steps/s.
+per-cell COM, forces, tick.  Paths on the same input:
+  * plain FP64, static work-queue order -- the rebuilt reference path;
+  * FP32 cell-relative forces, Kahan-compensated tile sums, static order;
+  * the same FP32 path with the learned work-queue order (the cost model in
+    barnes-hut_amd/costmodels/clusters256.npz; --cost-model none disables).
+`value` is the FP32 + learned-order path's whole-job throughput.
+
+Secondary lines (one untimed-in-`value` measurement each): 512 independent
+systems stepped by one graph launch per step (static vs learned order), and
+the reference's own logged configuration (N = 10M uniform in [0,499)^2, 120 x
+120 grid, FP64, G = 1, dt = 0.005) against its published 166.837 s/step.
 
 Prints one JSON line.
 """
@@ -180,6 +185,57 @@ def cpu_baseline(d, G, budget_s=12.0):
             "seconds_per_step": t_step}
 
 
+def batched_line(steps, model_path, seed=0):
+    """512 independent uniform systems (N log-uniform [1024, 262144], grid
+    side ceil(sqrt(N/16))), FP32, one graph launch per step."""
+    from barneshut import costmodel, inputs
+    from barneshut.batched import BatchedSystems
+    sizes, grids = inputs.batched_sizes(512, 1024, 262144, seed=seed)
+    data = inputs.batched_uniform(sizes, seed=seed)
+    out = {"systems": 512, "particles": int(sizes.sum()), "cells": int((grids.astype(np.int64) ** 2).sum())}
+    law = (GC, 1e-12, 0.0)
+    for name in ("static", "learned"):
+        model = None
+        if name == "learned":
+            if not model_path or not os.path.exists(model_path):
+                continue
+            import torch
+            model = costmodel.CostModel.load(model_path, device=torch.device("cuda"))
+        b = BatchedSystems(sizes, grids, data, law, "fp32", DT, graph=True, model=model)
+        sec = b.run(steps, warmup=1)
+        out[name] = {"ms_per_step": sec * 1e3, "value": float(sizes.sum()) / sec / 1e6}
+        b.close()
+    return out
+
+
+def reference_config_line(steps):
+    """The reference's logged run (python-bh/data/frontera/nbody_out_singlegpu.dat):
+    N = 10,000,000 uniform in [0,499)^2 (bin/gen_input.py), 120 x 120 grid,
+    G = 1, dt = 0.005, FP64, one step per run; 166.837 s/step published."""
+    from barneshut import inputs
+    from barneshut.engine import Engine
+    n, G = 10_000_000, 120
+    d = inputs.reference_normal(n, seed=1)
+    e = Engine([n], [G], tuning=dict(p=1, yspan=128, waves_per_cu=16))
+    e.load(d)
+    import torch
+    e.step(1.0, 0.0, 0.0, "fp64", 0.005)
+    e.check()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.step(1.0, 0.0, 0.0, "fp64", 0.005)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    e.check()
+    e.close()
+    published = 166.83713   # s/step, nbody_out_singlegpu.dat:35
+    return {"n": n, "grid": G, "precision": "fp64", "ms_per_step": sec * 1e3,
+            "value": n / sec / 1e6, "published_s_per_step": published,
+            "published_value": n / published / 1e6, "speedup_vs_published": published / sec,
+            "published_hardware": "one GPU of the reference's Frontera runs (model not recorded)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -195,10 +251,15 @@ def main():
     ap.add_argument("--p64", type=int, default=1, help="FP64 path: particles per lane")
     ap.add_argument("--autotune", action="store_true",
                     help="pick each path's launch shape with barneshut.autotune first")
-    ap.add_argument("--cost-model", default=None, help="trained cost-model file (.npz)")
+    ap.add_argument("--cost-model", default=os.path.join(PKG, "costmodels", "clusters256.npz"),
+                    help="trained cost-model file (.npz) or 'none'")
+    ap.add_argument("--batched-model", default=os.path.join(PKG, "costmodels", "batched512.npz"))
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-errors", action="store_true")
+    ap.add_argument("--skip-extra", action="store_true",
+                    help="skip the batched and reference-config lines")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -226,64 +287,66 @@ def main():
     tuning = dict(p=args.p, yspan=args.yspan, waves_per_cu=args.waves_per_cu)
     tuning64 = dict(p=args.p64, yspan=args.yspan, waves_per_cu=args.waves_per_cu)
 
+    def measure(precision, tun, order_model=None):
+        eng = Engine([args.n], [G], tuning=tun)
+        eng.load(d)
+        if args.autotune:
+            from barneshut import autotune
+            eng.create_tree()
+            eng.summarize()
+            tun = autotune.tune(eng, (GC, EPS2, R2CUT), precision, log=log)["tuning"]
+            eng.load(d)
+        order_fn = None
+        if order_model is not None:
+            order_buf = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
+
+            def order_fn():
+                eng.ctx.learned_order(order_model.params, order_buf)
+                return order_buf
+        sec, kms, near, far = run_path(eng, precision, args.steps, args.warmup, order_fn)
+        n_items = int(eng.ctx.stats()[2])
+        eng.close()
+        flops = FLOP_PER_INTERACTION * (near + far).astype(np.float64)
+        return dict(seconds_per_step=sec, value=args.n / sec / 1e6, kms=kms,
+                    force_kernel_ms=float(kms.mean()), near=near, far=far,
+                    flop_per_launch=float(flops.mean()),
+                    achieved=float(flops.sum() / (kms.sum() * 1e-3) / 1e12), tuning=tun,
+                    work_items=n_items)
+
     res64 = None
     if not args.skip_fp64:
         log("fp64 plain path")
-        e64 = Engine([args.n], [G], tuning=tuning64)
-        e64.load(d)
-        if args.autotune:
-            from barneshut import autotune
-            e64.create_tree()
-            e64.summarize()
-            tuning64 = autotune.tune(e64, (GC, EPS2, R2CUT), "fp64", log=log)["tuning"]
-            e64.load(d)
-        s64, k64, near64, far64 = run_path(e64, "fp64", args.steps, args.warmup)
-        f64 = FLOP_PER_INTERACTION * (near64 + far64).astype(np.float64)
-        res64 = dict(seconds_per_step=s64, value=args.n / s64 / 1e6,
-                     force_kernel_ms=float(k64.mean()),
-                     achieved=float(f64.sum() / (k64.sum() * 1e-3) / 1e12))
-        log(f"fp64: {s64 * 1e3:.1f} ms/step, force kernel {k64.mean():.1f} ms")
-        e64.close()
-        del e64
-
-    log("fp32 path")
-    e32 = Engine([args.n], [G], tuning=tuning)
-    e32.load(d)
-    if args.autotune:
-        from barneshut import autotune
-        e32.create_tree()
-        e32.summarize()
-        tuning = autotune.tune(e32, (GC, EPS2, R2CUT), "fp32", log=log)["tuning"]
-        e32.load(d)
-    order_fn = None
-    sched = "static"
-    if args.cost_model:
+        res64 = measure("fp64", tuning64)
+        log(f"fp64: {res64['seconds_per_step'] * 1e3:.1f} ms/step, "
+            f"force kernel {res64['force_kernel_ms']:.1f} ms")
+    res_static = None
+    if not args.skip_static:
+        log("fp32 path, static order")
+        res_static = measure("fp32", tuning)
+        log(f"fp32 static: {res_static['seconds_per_step'] * 1e3:.1f} ms/step, "
+            f"force kernel {res_static['force_kernel_ms']:.1f} ms")
+    model = None
+    if args.cost_model and args.cost_model != "none" and os.path.exists(args.cost_model):
         from barneshut import costmodel
-        model = costmodel.CostModel.load(args.cost_model, device=e32.dev)
-        order_buf = torch.empty(e32.ctx.query(4), dtype=torch.int32, device=e32.dev)
-
-        def order_fn():
-            e32.ctx.learned_order(model.params, order_buf)
-            return order_buf
+        model = costmodel.CostModel.load(args.cost_model, device=torch.device("cuda"))
+    if model is not None:
+        log("fp32 path, learned order")
+        res = measure("fp32", tuning, model)
         sched = "learned"
-    s32, k32v, near32, far32 = run_path(e32, "fp32", args.steps, args.warmup, order_fn)
-    k32 = float(k32v.mean())
-    log(f"fp32 ({sched}): {s32 * 1e3:.1f} ms/step, force kernel {k32:.1f} ms")
-    n_items = int(e32.ctx.stats()[2])
-    e32.close()
-    del e32
+        log(f"fp32 learned: {res['seconds_per_step'] * 1e3:.1f} ms/step, "
+            f"force kernel {res['force_kernel_ms']:.1f} ms")
+    else:
+        res = res_static if res_static is not None else measure("fp32", tuning)
+        sched = "static"
 
-    f32 = FLOP_PER_INTERACTION * (near32 + far32).astype(np.float64)
-    flops = float(f32.mean())
-    achieved = float(f32.sum() / (k32v.sum() * 1e-3) / 1e12)
     out = {
         "metric": METRIC,
-        "value": args.n / s32 / 1e6,
+        "value": res["value"],
         "unit": "Mparticle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": s32 * 1e3,
+        "ms_per_step": res["seconds_per_step"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -293,24 +356,37 @@ def main():
                                "1024x1024 grid",
                    "n": args.n, "grid": G, "clusters": N_CLUSTERS, "sigma": list(SIGMA),
                    "mass": "U[1,2]", "G": GC, "dt": DT, "eps2": EPS2, "r2_cut": R2CUT,
-                   "seed": args.seed, "schedule": sched, "tuning": tuning,
-                   "work_items": n_items},
-        "interactions_per_step": {"near": float(near32.mean()), "far": float(far32.mean())},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": TRAFFIC,
-                     "kernel": "k_force32<P,0,1>", "force_kernel_ms": k32,
-                     "flop_per_launch": flops},
+                   "seed": args.seed, "schedule": sched, "cost_model":
+                       os.path.relpath(args.cost_model, ROOT) if model is not None else None,
+                   "tuning": res["tuning"], "work_items": res["work_items"]},
+        "interactions_per_step": {"near": float(res["near"].mean()),
+                                  "far": float(res["far"].mean())},
+        "roofline": {"bound": "valu", "achieved": res["achieved"], "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": res["achieved"] / FP32_PEAK_TFLOPS,
+                     "traffic": TRAFFIC, "kernel": "k_force32<P,0,1>",
+                     "force_kernel_ms": res["force_kernel_ms"],
+                     "flop_per_launch": res["flop_per_launch"]},
     }
+    if res_static is not None:
+        out["fp32_static"] = {"value": res_static["value"],
+                              "ms_per_step": res_static["seconds_per_step"] * 1e3,
+                              "force_kernel_ms": res_static["force_kernel_ms"],
+                              "achieved_tflops": res_static["achieved"]}
+        out["learned_vs_static"] = res_static["seconds_per_step"] / res["seconds_per_step"]
     if res64 is not None:
-        a64 = res64["achieved"]
         out["fp64_plain"] = {"value": res64["value"], "ms_per_step": res64["seconds_per_step"] * 1e3,
-                             "tuning": tuning64,
-                             "force_kernel_ms": res64["force_kernel_ms"],
-                             "achieved_tflops": a64, "frac_of_fp64_peak": a64 / FP64_PEAK_TFLOPS}
-        out["speedup_vs_fp64"] = res64["seconds_per_step"] / s32
+                             "tuning": res64["tuning"], "force_kernel_ms": res64["force_kernel_ms"],
+                             "achieved_tflops": res64["achieved"],
+                             "frac_of_fp64_peak": res64["achieved"] / FP64_PEAK_TFLOPS}
+        out["speedup_vs_fp64"] = res64["seconds_per_step"] / res["seconds_per_step"]
     if not args.skip_errors:
         log("force errors")
         out["force_rel_err"] = force_errors(d, G, Engine)
+    if not args.skip_extra:
+        log("batched systems")
+        out["batched_512"] = batched_line(args.steps, args.batched_model)
+        log("reference configuration")
+        out["reference_config"] = reference_config_line(3)
     if not args.skip_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(d, G)

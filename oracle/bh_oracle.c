@@ -15,7 +15,7 @@
  *   GPU reference:  eps2 = 0, r2_cut = 0   (kernels/grid_decomposition/gpu/grid.py:224-230)
  *   CPU reference:  eps2 = 0, r2_cut = 1e-5 (kernels/gravity/cpu_numba.py:36,44,59,67,81,89)
  *
- * Build: see oracle/Makefile (gcc -O2 -fopenmp, -ffp-contract=off so every
+ * Build: see oracle/Makefile (gcc -O3 -fopenmp, -ffp-contract=off so every
  * multiply and add rounds separately, like numpy).
  */
 #include <math.h>
@@ -27,7 +27,7 @@
 #endif
 
 /* ------------------------------------------------------------------ */
-/* A1: grid sizing.                                                     */
+/* Grid sizing.                                                        */
 /* python-bh/barneshut/implementations/base.py:221-231 (LEAF_OCCUPANCY  */
 /* = 0.7 at :13) and kernels/helpers/grid.py:5-8 (next_perfect_square). */
 /* ------------------------------------------------------------------ */
@@ -49,7 +49,7 @@ int bho_grid_dim(int64_t n, int64_t ppb)
     return (int)sqrt(nleaves);
 }
 
-/* A1: bounding box, helpers/grid.py:10-26; step, base.py:237.
+/* Bounding box, helpers/grid.py:10-26; step, base.py:237.
  * out = {min_x, min_y, max_x, max_y, step} with the shorter edge's max
  * extended so the box is square (grid.py:19-22). */
 void bho_bbox(int64_t n, const double *px, const double *py, int grid_dim, double *out)
@@ -74,7 +74,7 @@ void bho_bbox(int64_t n, const double *px, const double *py, int grid_dim, doubl
     out[4] = (mxx - mnx) / (double)grid_dim;   /* base.py:237 */
 }
 
-/* A2: cell ids, implementations/sequential.py:32-34:
+/* Cell ids, implementations/sequential.py:32-34:
  * g = clip(floor((p - min) / step), 0, G-1); key = gx*G + gy (x-major). */
 void bho_place(int64_t n, const double *px, const double *py, double min_x, double min_y,
                double step, int G, int32_t *gx, int32_t *gy, int32_t *key)
@@ -93,7 +93,7 @@ void bho_place(int64_t n, const double *px, const double *py, double min_x, doub
     }
 }
 
-/* A4: permutation of implementations/sequential.py:37,
+/* Permutation of implementations/sequential.py:37,
  *   particles.view('f8,...').sort(order=['f7','f8'], axis=0, kind='stable')
  * numpy compares the named fields first, then every remaining field in
  * dtype order: px, py, mass, vx, vy, ax, ay, pid (internals/particle.py:3-9).
@@ -122,7 +122,7 @@ void bho_sort_perm(int64_t n, const int32_t *key, const double *const *fields, i
     qsort(perm, (size_t)n, sizeof(int64_t), cmp_rows);
 }
 
-/* A3: per-cell counts and exclusive offsets in x-major key order
+/* Per-cell counts and exclusive offsets in x-major key order
  * (the np.unique ranges of sequential.py:39-50). offsets has ncells+1. */
 void bho_counts(int64_t n, const int32_t *key, int64_t ncells, int64_t *counts, int64_t *offsets)
 {
@@ -143,7 +143,7 @@ static inline void cs_add(csum *a, double v)
 }
 static inline double cs_val(const csum *a) { return a->s + a->c; }
 
-/* A5: per-cell centre of mass, internals/cloud.py:9-17 (calc_COM) and
+/* Per-cell centre of mass, internals/cloud.py:9-17 (calc_COM) and
  * gpu/grid.py:78-114 (g_summarize): M = sum m, X = sum m*x / M,
  * Y = sum m*y / M; empty cell -> (0, 0, 0). Sums are compensated so the
  * oracle is more accurate than either reference variant.
@@ -179,7 +179,7 @@ static inline double law_w(double s, double eps2, double r2_cut)
     return 1.0 / (s + eps2);
 }
 
-/* A6/A7: accelerations of the grid method (App. "plain method"):
+/* Accelerations of the grid method:
  *   near: every particle j != i of the 3x3 cell block around i's cell
  *         (gpu/grid.py:139-148, d_is_neighbor :187-192; CPU
  *          seq/evaluation.py:73-109 with helpers/grid.py:28-48);
@@ -295,7 +295,7 @@ void bho_direct(int64_t n, const double *x, const double *y, const double *m,
     }
 }
 
-/* A8: symplectic-Euler tick, gpu/grid.py:159-174 (g_tick_particles):
+/* Symplectic-Euler tick, gpu/grid.py:159-174 (g_tick_particles):
  * v += a*dt; a = 0; x += v*dt. Separate multiply and add (no FMA). */
 void bho_tick(int64_t n, double *x, double *y, double *vx, double *vy, double *ax, double *ay,
               double dt)

@@ -37,7 +37,7 @@ def _ties(n, seed=3):
 
 
 CASES = [
-    ("c1_uniform", lambda: inputs.uniform(16384, seed=1), 32),
+    ("uniform16k_g32", lambda: inputs.uniform(16384, seed=1), 32),
     ("ref_normal", lambda: inputs.reference_normal(20000, seed=2), orc.grid_dim(20000, 1000)),
     ("ties_G24", lambda: _ties(30000), 24),
     ("big_cells_G2", lambda: inputs.gaussian_mixture(9000, 3, (0.05, 0.2), seed=4), 2),
@@ -139,7 +139,7 @@ def test_com_fp64():
 
 
 LAWS = [("gpu_ref", 1.0, 0.0, 0.0), ("cpu_ref_cut", 1.0, 0.0, 1e-5),
-        ("c1_soft", 1.0, 1e-6, 0.0), ("G2_soft_cut", 2.5, 1e-6, 1e-5)]
+        ("softened_eps1e-3", 1.0, 1e-6, 0.0), ("G2_soft_cut", 2.5, 1e-6, 1e-5)]
 
 
 @pytest.mark.parametrize("law", LAWS, ids=[l[0] for l in LAWS])

@@ -1,7 +1,8 @@
 """Autotuner for the force kernel's launch shape.
 
-Candidates are (p, yspan, waves_per_cu): particles per lane, the largest row
-span of a work item, and persistent waves per compute unit.  Each candidate
+Candidates are (p, yspan, waves_per_cu, tail_split): particles per lane, the
+largest row span of a work item, persistent waves per compute unit, and the
+far-field slices each FP32 item is split into.  Each candidate
 is timed with HIP events on the engine's current state (after create_tree and
 summarize).  For FP32 paths a candidate is admitted only if its accelerations
 stay within `max_rel_err` (normwise) of an FP64 pass on the same state: the
@@ -14,7 +15,8 @@ import itertools
 import json
 import os
 
-DEFAULT_CANDIDATES = dict(p=(1, 2, 4), yspan=(16, 32, 64, 128), waves_per_cu=(8, 16))
+DEFAULT_CANDIDATES = dict(p=(1, 2, 4), yspan=(128, 512), waves_per_cu=(16, 32),
+                          tail_split=(1, 8))
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "bh_mi355x_autotune.json")
 
 
@@ -71,8 +73,10 @@ def tune(eng, law=(1.0, 0.0, 0.0), precision="fp32", candidates=None, max_rel_er
         den = torch.sqrt((ref[0] ** 2 + ref[1] ** 2).sum())
     table = []
     best = None
-    for p, ys, w in itertools.product(cand["p"], cand["yspan"], cand["waves_per_cu"]):
-        tun = dict(p=int(p), yspan=int(ys), waves_per_cu=int(w))
+    splits = cand["tail_split"] if precision != "fp64" else (1,)
+    for p, ys, w, sp in itertools.product(cand["p"], cand["yspan"], cand["waves_per_cu"], splits):
+        tun = dict(p=int(p), yspan=int(ys), waves_per_cu=int(w), tail_split=int(sp),
+                   tail_items=(1 << 20) if sp > 1 else 0)
         eng.ctx.set_tuning(**tun)
         eng.ctx.build_items()
         ms = _time_force(eng, law, precision, reps=reps)

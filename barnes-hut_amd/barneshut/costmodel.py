@@ -89,6 +89,8 @@ def collect(eng, steps, law, precision="fp32", dt=0.0, order_fn=None, log=None):
     import torch
     feats, costs = [], []
     cap = eng.ctx.query(4)
+    saved = eng.ctx.get_tuning()
+    eng.ctx.set_tuning(**dict(saved, tail_split=1, tail_items=0))   # durations of whole items
     fbuf = torch.empty(cap * NF, dtype=torch.float32, device=eng.dev)
     for s in range(steps):
         eng.create_tree()
@@ -107,6 +109,7 @@ def collect(eng, steps, law, precision="fp32", dt=0.0, order_fn=None, log=None):
             eng.timestep(dt)
         if log:
             log(f"collect step {s}: {n} items, median {np.median(c)} ticks")
+    eng.ctx.set_tuning(**saved)
     return np.concatenate(feats), np.concatenate(costs)
 
 

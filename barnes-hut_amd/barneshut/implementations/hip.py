@@ -23,7 +23,8 @@ Extra keys, section [hip]:
   grid_dim    explicit grid side (0: from max_particles_per_box)
   schedule    static (default) | learned
   cost_model  .npz file of a trained cost model (schedule = learned)
-  p, yspan, waves_per_cu   force-kernel launch shape; autotune = True to tune
+  p, yspan, waves_per_cu, tail_split, tail_items   force-kernel launch shape
+              (autotune = True to tune)
 """
 import numpy as np
 
@@ -43,9 +44,11 @@ class HipBarnesHut(BaseBarnesHut):
         self.schedule = Config.get("hip", "schedule", fallback="static")
         self.cost_model_path = Config.get("hip", "cost_model", fallback="")
         self.autotune = Config.get("hip", "autotune", fallback="False") == "True"
-        self.tuning = dict(p=int(Config.get("hip", "p", fallback="2")),
-                           yspan=int(Config.get("hip", "yspan", fallback="128")),
-                           waves_per_cu=int(Config.get("hip", "waves_per_cu", fallback="16")))
+        self.tuning = dict(p=int(Config.get("hip", "p", fallback="4")),
+                           yspan=int(Config.get("hip", "yspan", fallback="512")),
+                           waves_per_cu=int(Config.get("hip", "waves_per_cu", fallback="32")),
+                           tail_split=int(Config.get("hip", "tail_split", fallback="8")),
+                           tail_items=int(Config.get("hip", "tail_items", fallback="1048576")))
         self.device = int(Config.get("hip", "device", fallback="0"))
         self.engine = None
         self.model = None

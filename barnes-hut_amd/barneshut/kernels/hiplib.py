@@ -16,9 +16,10 @@ _CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abs
 LIB_PATH = os.path.join(_CSRC, "libbh_hip.so")
 
 BH_OK, BH_EINVAL, BH_EHIP, BH_EDEVICE, BH_ESTATE = 0, 1, 2, 3, 4
-PREC_FP64, PREC_FP32_KAHAN, PREC_FP32_KAHAN_TERM, PREC_FP32_PLAIN = 0, 1, 2, 3
+PREC_FP64, PREC_FP32_KAHAN, PREC_FP32_KAHAN_TERM, PREC_FP32_PLAIN, PREC_FP32_MFMA = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32_KAHAN, "fp32_kahan": PREC_FP32_KAHAN,
-              "fp32_kahan_term": PREC_FP32_KAHAN_TERM, "fp32_plain": PREC_FP32_PLAIN}
+              "fp32_kahan_term": PREC_FP32_KAHAN_TERM, "fp32_plain": PREC_FP32_PLAIN,
+              "fp32_mfma": PREC_FP32_MFMA}
 (EXPORT_COUNTS, EXPORT_OFFSETS, EXPORT_NEPREFIX, EXPORT_PERM, EXPORT_KEYS, EXPORT_COM,
  EXPORT_BBOX, EXPORT_ITEMS, EXPORT_COST, EXPORT_KEYS_IN) = range(10)
 MLP_NPARAM = 1377
@@ -53,7 +54,8 @@ class BhLaw(ctypes.Structure):
 
 
 class BhTuning(ctypes.Structure):
-    _fields_ = [("p", ctypes.c_int32), ("yspan", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32)]
+    _fields_ = [("p", ctypes.c_int32), ("yspan", ctypes.c_int32), ("waves_per_cu", ctypes.c_int32),
+                ("tail_split", ctypes.c_int32), ("tail_items", ctypes.c_int32)]
 
 
 _lib = None
@@ -182,14 +184,15 @@ class Context:
         self.n = n.copy()
         self.grid_dim = g.copy()
 
-    def set_tuning(self, p=2, yspan=16, waves_per_cu=16):
-        t = BhTuning(int(p), int(yspan), int(waves_per_cu))
+    def set_tuning(self, p=4, yspan=512, waves_per_cu=32, tail_split=8, tail_items=1 << 20):
+        t = BhTuning(int(p), int(yspan), int(waves_per_cu), int(tail_split), int(tail_items))
         check(self.L.bh_set_tuning(self.h, ctypes.byref(t)))
 
     def get_tuning(self):
         t = BhTuning()
         check(self.L.bh_get_tuning(self.h, ctypes.byref(t)))
-        return dict(p=t.p, yspan=t.yspan, waves_per_cu=t.waves_per_cu)
+        return dict(p=t.p, yspan=t.yspan, waves_per_cu=t.waves_per_cu, tail_split=t.tail_split,
+                    tail_items=t.tail_items)
 
     def set_cost_log(self, on: bool):
         check(self.L.bh_set_cost_log(self.h, 1 if on else 0))

@@ -75,7 +75,9 @@ struct bh_ctx_s {
     int32_t* cost_scratch = nullptr;
     float* feat = nullptr;
     float* pred = nullptr;
-    bh_tuning tune = {2, 16, 16};
+    bh_tuning tune = {4, 512, 32, 8, 1 << 20};
+    double* part = nullptr;
+    size_t part_cap = 0;
     hipStream_t cap_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -110,6 +112,7 @@ static int dalloc(Ctx* c, T** p, int64_t count)
 }
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+static int ensure_part(Ctx* c);
 static inline unsigned grid_for(int64_t n, int bs, int64_t cap = 1 << 20)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, bs), cap));
@@ -151,6 +154,7 @@ int bh_destroy(void* ctx)
     (void)hipSetDevice(c->device);
     (void)hipDeviceSynchronize();
     free_all(c);
+    if (c->part) (void)hipFree(c->part);
     if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
     if (c->ev_out) (void)hipEventDestroy(c->ev_out);
@@ -232,6 +236,23 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     HIP_TRY(hipMemset(c->offsets, 0, sizeof(int32_t) * (C + 1)));
     HIP_TRY(hipMemset(c->neprefix, 0, sizeof(int32_t) * (C + 1)));
     HIP_TRY(hipDeviceSynchronize());
+    return ensure_part(c);
+}
+
+// Scratch of the split force pass: N x tail_split x 2 doubles.
+static int ensure_part(Ctx* c)
+{
+    size_t need = (c->tune.tail_split > 1 && c->tune.tail_items > 0 && c->N > 0)
+                      ? (size_t)c->N * c->tune.tail_split * 2 * sizeof(double)
+                      : 0;
+    if (need > c->part_cap) {
+        HIP_TRY(hipDeviceSynchronize());
+        if (c->part) (void)hipFree(c->part);
+        c->part = nullptr;
+        c->part_cap = 0;
+        HIP_TRY(hipMalloc(&c->part, need));
+        c->part_cap = need;
+    }
     return BH_OK;
 }
 
@@ -243,8 +264,12 @@ int bh_set_tuning(void* ctx, const bh_tuning* t)
     if (t->yspan < 1 || t->yspan > 1 << 20) return fail(BH_EINVAL, "tuning.yspan must be >= 1");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 64)
         return fail(BH_EINVAL, "tuning.waves_per_cu must be in [1, 64]");
+    if (t->tail_split < 1 || t->tail_split > 8)
+        return fail(BH_EINVAL, "tuning.tail_split must be in [1, 8]");
+    if (t->tail_items < 0 || t->tail_items > (1 << 20))
+        return fail(BH_EINVAL, "tuning.tail_items must be in [0, 2^20]");
     c->tune = *t;
-    return BH_OK;
+    return ensure_part(c);
 }
 
 int bh_get_tuning(void* ctx, bh_tuning* t)
@@ -447,7 +472,7 @@ int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32
     int r = check_parts(p, "particles");
     if (r) return r;
     if (!law) return fail(BH_EINVAL, "law is NULL");
-    if (precision < 0 || precision > 3) return fail(BH_EINVAL, "precision must be in [0, 3]");
+    if (precision < 0 || precision > 4) return fail(BH_EINVAL, "precision must be in [0, 4]");
     if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
         return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
     if (!order) {
@@ -464,7 +489,7 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     int r = check_parts(p, "particles");
     if (r) return r;
     if (!law) return fail(BH_EINVAL, "law is NULL");
-    if (precision < 0 || precision > 3) return fail(BH_EINVAL, "precision must be in [0, 3]");
+    if (precision < 0 || precision > 4) return fail(BH_EINVAL, "precision must be in [0, 4]");
     if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
         return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
     hipStream_t st = (hipStream_t)stream;
@@ -491,10 +516,20 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     fa.eps2 = law->eps2;
     fa.r2cut = law->r2_cut;
     fa.cost_log = c->cost_log_on ? c->cost_log : nullptr;
+    const bool fp32v = precision >= 1 && precision <= 3;
+    fa.split = (fp32v && c->tune.tail_split > 1 && c->tune.tail_items > 0) ? c->tune.tail_split : 1;
+    fa.n_tail = fa.split > 1 ? c->tune.tail_items : 0;
+    fa.part = c->part;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
     void* args[] = {&fa};
     unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
+    if (fa.split > 1) {
+        const int pslots = 64 * c->tune.p;
+        hipLaunchKernelGGL(k_force_combine, dim3(cdiv((int64_t)fa.n_tail * pslots, 256)), dim3(256),
+                           0, st, (const Item*)c->items, order, (const int32_t*)c->stats, fa.split,
+                           fa.n_tail, pslots, (const double*)c->part, p->ax, p->ay);
+    }
     LAUNCH_CHECK("evaluate");
     return BH_OK;
 }

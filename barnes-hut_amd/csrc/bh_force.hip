@@ -323,11 +323,21 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
     __shared__ NearE ntile[BH_WAVE];
     const int lane = threadIdx.x;
     const int nitems = A.stats[ST_NITEMS];
+    const int S = A.split > 1 ? A.split : 1;
+    const int nt = S > 1 ? min(A.n_tail, nitems) : 0;
+    const int nu = nitems - nt;
+    const int total = nu + nt * S;
     for (;;) {
         const int q = next_item(A.stats, lane);
-        if (q >= nitems) break;
+        if (q >= total) break;
+        int pos = q, part = -1;
+        if (q >= nu) {
+            const int r = q - nu;
+            pos = nu + r / S;
+            part = r - (r / S) * S;
+        }
         const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
-        const int itx = A.order ? uniform_int(A.order[q]) : q;
+        const int itx = A.order ? uniform_int(A.order[pos]) : pos;
         if ((unsigned)itx >= (unsigned)nitems) continue;   // malformed order entry
         const Item it = A.items[itx];
         const SysDev& d = A.sys[it.sys];
@@ -365,8 +375,8 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
             Sx[k] = Sy[k] = Cx[k] = Cy[k] = tx[k] = ty[k] = 0.0f;
         }
 
-        // near field (separations in FP64, rounded once to FP32)
-        for (int dx = -1; dx <= 1; ++dx) {
+        // near field (separations in FP64, rounded once to FP32); part 0 of a split item
+        for (int dx = -1; dx <= (part <= 0 ? 1 : -2); ++dx) {
             const int xc = cx + dx;
             if (xc < 0 || xc >= G) continue;
             const int64_t cl = coff + (int64_t)xc * G;
@@ -424,8 +434,13 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
             }
         }
 
-        // far field
-        const int ne0 = A.neprefix[coff], ne1 = A.neprefix[coff + (int64_t)G * G];
+        // far field (a split item's part covers one 64-aligned slice of the list)
+        int ne0 = A.neprefix[coff], ne1 = A.neprefix[coff + (int64_t)G * G];
+        if (part >= 0) {
+            const int chunk = (((ne1 - ne0 + S - 1) / S) + BH_WAVE - 1) & ~(BH_WAVE - 1);
+            ne0 = ne0 + part * chunk;
+            ne1 = min(ne1, ne0 + chunk);
+        }
         int mlo[3], mhi[3];
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
@@ -527,6 +542,274 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                     rx = (double)Sx[k] - (double)Cx[k];
                     ry = (double)Sy[k] - (double)Cy[k];
                 }
+                if (part < 0) {
+                    A.ax[gi[k]] += scale * rx;
+                    A.ay[gi[k]] += scale * ry;
+                } else {
+                    const int64_t o = ((int64_t)gi[k] * S + part) * 2;
+                    A.part[o] = scale * rx;
+                    A.part[o + 1] = scale * ry;
+                }
+            }
+        }
+        if (A.cost_log && lane == 0 && part < 0) A.cost_log[itx] = (long long)wall_clock64() - t0;
+    }
+}
+
+// Adds the parts of the split tail items, in part order (deterministic).
+__global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ items,
+                                                       const int32_t* __restrict__ order,
+                                                       const int32_t* __restrict__ stats,
+                                                       int split, int n_tail, int pslots,
+                                                       const double* __restrict__ part,
+                                                       double* __restrict__ ax,
+                                                       double* __restrict__ ay)
+{
+    const int nitems = stats[ST_NITEMS];
+    const int nt = min(n_tail, nitems);
+    const int nu = nitems - nt;
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = (int)(g / pslots), slot = (int)(g - (int64_t)j * pslots);
+    if (j >= nt) return;
+    const int pos = nu + j;
+    const int itx = order ? order[pos] : pos;
+    if ((unsigned)itx >= (unsigned)nitems) return;
+    const Item it = items[itx];
+    if (slot >= it.np) return;
+    const int64_t gi = (int64_t)it.p0 + slot;
+    double sx = 0.0, sy = 0.0;
+    for (int s = 0; s < split; ++s) {
+        const int64_t o = (gi * split + s) * 2;
+        sx += part[o];
+        sy += part[o + 1];
+    }
+    ax[gi] += sx;
+    ay[gi] += sy;
+}
+
+// ---------------------------------------------------------------- FP32 + matrix cores
+// Same decomposition and coordinates as k_force32, with the far-field sum
+// split as  sum_k M_k r_k (a_k - u) = sum_k r_k (M_k a_k) - u sum_k r_k M_k,
+// r_k = 1/(|a_k - u|^2 + eps2).  Per COM entry the tile holds
+// B_k = (M a_x, M a_y, M, 0); every lane computes r_k for its particle (2 sub,
+// r^2 as mul+fma, v_rcp_f32) and v_mfma_f32_4x4x1f32 adds B_k * r_k into the
+// lane's own column: with A from lane 4b+c and B from lane l, lane l's
+// register c receives A(lane 4*(l/4) + c) * B(lane l).  So lane 4b+c supplies
+// component c of B_k as the A operand and its own r_k as the B operand.  The matrix unit accumulates one
+// 64-entry tile (FP32); tiles are Kahan-summed; the final combination
+// sum r M a - u sum r M is done in FP64.  Near field as in k_force32.
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+template <int P, int LAW>
+__global__ __launch_bounds__(64) void k_force32m(ForceArgs A)
+{
+    __shared__ float2 atile[BH_WAVE];          // (a_x, a_y) relative to the item origin
+    __shared__ float btile[BH_WAVE][4];        // (M a_x, M a_y, M, 0)
+    __shared__ int32_t ktile[BH_WAVE];         // kx << 16 | ky
+    __shared__ NearE ntile[BH_WAVE];
+    const int lane = threadIdx.x;
+    const int bc = lane & 3;
+    const int nitems = A.stats[ST_NITEMS];
+    for (;;) {
+        const int q = next_item(A.stats, lane);
+        if (q >= nitems) break;
+        const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
+        const int itx = A.order ? uniform_int(A.order[q]) : q;
+        if ((unsigned)itx >= (unsigned)nitems) continue;
+        const Item it = A.items[itx];
+        const SysDev& d = A.sys[it.sys];
+        const int G = d.G;
+        const int64_t coff = d.c_off;
+        const int cx = it.cx;
+        const int oy = (it.y0 + it.y1) >> 1;
+        const int ylo = max(it.y0 - 1, 0), yhi = min(it.y1 + 1, G - 1);
+        const double step = d.step;
+        const double inv_step = 1.0 / step;
+        const float eps2c = (float)(A.eps2 * inv_step * inv_step);
+        const float r2c = (float)(A.r2cut * inv_step * inv_step);
+
+        double xi[P], yi[P];
+        float ux[P], uy[P], Sx[P], Sy[P], Cx[P], Cy[P], tx[P], ty[P];
+        float Fx[P], Fy[P], Fm[P], Gx[P], Gy[P], Gm[P];   // far sums (Kahan) of r M a_x, r M a_y, r M
+        int cy[P], gi[P];
+        bool act[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            int slot = lane + BH_WAVE * k;
+            act[k] = slot < it.np;
+            gi[k] = it.p0 + slot;
+            xi[k] = 0.0;
+            yi[k] = 0.0;
+            ux[k] = 0.0f;
+            uy[k] = 0.0f;
+            cy[k] = -100000;
+            if (act[k]) {
+                xi[k] = A.x[gi[k]];
+                yi[k] = A.y[gi[k]];
+                cy[k] = (int)(A.key_sorted[gi[k]] - coff) - cx * G;
+                ux[k] = (float)((xi[k] - d.min_x) / step - (double)cx);
+                uy[k] = (float)((yi[k] - d.min_y) / step - (double)oy);
+            }
+            Sx[k] = Sy[k] = Cx[k] = Cy[k] = tx[k] = ty[k] = 0.0f;
+            Fx[k] = Fy[k] = Fm[k] = Gx[k] = Gy[k] = Gm[k] = 0.0f;
+        }
+
+        // near field (separations in FP64, rounded once to FP32)
+        for (int dx = -1; dx <= 1; ++dx) {
+            const int xc = cx + dx;
+            if (xc < 0 || xc >= G) continue;
+            const int64_t cl = coff + (int64_t)xc * G;
+            const int pa = A.offsets[cl + ylo], pb = A.offsets[cl + yhi + 1];
+            for (int base = pa; base < pb; base += BH_WAVE) {
+                const int j = base + lane;
+                NearE e;
+                if (j < pb) {
+                    e.x = A.x[j];
+                    e.y = A.y[j];
+                    e.m = A.m[j];
+                    e.row = (int)(A.key_sorted[j] - cl);
+                    e.idx = j;
+                } else {
+                    e.x = 0.0;
+                    e.y = 0.0;
+                    e.m = 0.0;
+                    e.row = -100000;
+                    e.idx = -1;
+                }
+                ntile[lane] = e;
+                __syncthreads();
+                const int cnt = min(BH_WAVE, pb - base);
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const NearE f = ntile[jj];
+                    const float fm = (float)f.m;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const bool keep = (f.idx != gi[k]) && (abs(f.row - cy[k]) <= 1);
+                        const float wx = (float)((f.x - xi[k]) * inv_step);
+                        const float wy = (float)((f.y - yi[k]) * inv_step);
+                        const float s2 = fmaf(wx, wx, wy * wy);
+                        float t = fm * __builtin_amdgcn_rcpf(s2 + eps2c);
+                        if (LAW == 1) t = (s2 > r2c) ? t : 0.0f;
+                        t = keep ? t : 0.0f;
+                        tx[k] = fmaf(wx, t, tx[k]);
+                        ty[k] = fmaf(wy, t, ty[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < P; ++k) {
+                    kahan_add(Sx[k], Cx[k], tx[k]);
+                    kahan_add(Sy[k], Cy[k], ty[k]);
+                    tx[k] = 0.0f;
+                    ty[k] = 0.0f;
+                }
+                __syncthreads();
+            }
+        }
+
+        // far field
+        const int ne0 = A.neprefix[coff], ne1 = A.neprefix[coff + (int64_t)G * G];
+        int mlo[3], mhi[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int xc = cx + t - 1;
+            if (xc >= 0 && xc < G) {
+                const int64_t cl = coff + (int64_t)xc * G;
+                mlo[t] = A.neprefix[cl + ylo];
+                mhi[t] = A.neprefix[cl + yhi + 1];
+            } else {
+                mlo[t] = 0;
+                mhi[t] = 0;
+            }
+        }
+        Com32 nxt;
+        {
+            const int e = ne0 + lane;
+            if (e < ne1) nxt = A.l32[e];
+            else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+        }
+        for (int base = ne0; base < ne1; base += BH_WAVE) {
+            {
+                const int kx = nxt.kxy >> 16, ky = nxt.kxy & 0xffff;
+                const float ax_ = (float)(kx - cx) + nxt.fx;
+                const float ay_ = (float)(ky - oy) + nxt.fy;
+                atile[lane] = make_float2(ax_, ay_);
+                btile[lane][0] = nxt.M * ax_;
+                btile[lane][1] = nxt.M * ay_;
+                btile[lane][2] = nxt.M;
+                btile[lane][3] = 0.0f;
+                ktile[lane] = nxt.kxy;
+            }
+            __syncthreads();
+            {
+                const int e = base + BH_WAVE + lane;
+                if (e < ne1) nxt = A.l32[e];
+                else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+            }
+            const int b1 = base + BH_WAVE;
+            const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||
+                                meets(base, b1, mlo[2], mhi[2]);
+            mf4 acc[P];
+#pragma unroll
+            for (int k = 0; k < P; ++k) acc[k] = (mf4){0.0f, 0.0f, 0.0f, 0.0f};
+            if (!masked) {
+#pragma unroll
+                for (int j = 0; j < BH_WAVE; ++j) {
+                    const float2 a = atile[j];
+                    const float bj = btile[j][bc];
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const float wx = a.x - ux[k], wy = a.y - uy[k];
+                        float r;
+                        if (LAW == 0) {
+                            r = __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, eps2c)));
+                        } else {
+                            const float s2 = fmaf(wx, wx, wy * wy);
+                            r = (s2 > r2c) ? __builtin_amdgcn_rcpf(s2 + eps2c) : 0.0f;
+                        }
+                        acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(bj, r, acc[k], 0, 0, 0);
+                    }
+                }
+            } else {
+                for (int j = 0; j < BH_WAVE; ++j) {
+                    const float2 a = atile[j];
+                    const float bj = btile[j][bc];
+                    const int kxy = ktile[j];
+                    const bool colnear = abs((kxy >> 16) - cx) <= 1;
+                    const int ky = kxy & 0xffff;
+#pragma unroll
+                    for (int k = 0; k < P; ++k) {
+                        const bool keep = !(colnear && abs(ky - cy[k]) <= 1);
+                        const float wx = a.x - ux[k], wy = a.y - uy[k];
+                        float r;
+                        if (LAW == 0) {
+                            r = __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, eps2c)));
+                        } else {
+                            const float s2 = fmaf(wx, wx, wy * wy);
+                            r = (s2 > r2c) ? __builtin_amdgcn_rcpf(s2 + eps2c) : 0.0f;
+                        }
+                        r = keep ? r : 0.0f;
+                        acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(bj, r, acc[k], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+                kahan_add(Fx[k], Gx[k], acc[k][0]);
+                kahan_add(Fy[k], Gy[k], acc[k][1]);
+                kahan_add(Fm[k], Gm[k], acc[k][2]);
+            }
+            __syncthreads();
+        }
+
+        const double scale = A.Gc * inv_step;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+            if (act[k]) {
+                const double sm = (double)Fm[k] - (double)Gm[k];
+                const double fx = ((double)Fx[k] - (double)Gx[k]) - (double)ux[k] * sm;
+                const double fy = ((double)Fy[k] - (double)Gy[k]) - (double)uy[k] * sm;
+                const double rx = fx + ((double)Sx[k] - (double)Cx[k]);
+                const double ry = fy + ((double)Sy[k] - (double)Cy[k]);
                 A.ax[gi[k]] += scale * rx;
                 A.ay[gi[k]] += scale * ry;
             }
@@ -540,17 +823,19 @@ typedef void (*force_fn)(ForceArgs);
 
 #define F64(P, L) (force_fn)k_force64<P, L>
 #define F32(P, L, K) (force_fn)k_force32<P, L, K>
+#define F32M(P, L) (force_fn)k_force32m<P, L>
 
-// [precision: 0 fp64, 1..3 fp32 kahan mode 1,2,0][P index][law]
-static const force_fn g_force[4][3][2] = {
+// [precision: 0 fp64, 1..3 fp32 kahan mode 1,2,0, 4 fp32 matrix-core][P index][law]
+static const force_fn g_force[5][3][2] = {
     {{F64(1, 0), F64(1, 1)}, {F64(2, 0), F64(2, 1)}, {F64(4, 0), F64(4, 1)}},
     {{F32(1, 0, 1), F32(1, 1, 1)}, {F32(2, 0, 1), F32(2, 1, 1)}, {F32(4, 0, 1), F32(4, 1, 1)}},
     {{F32(1, 0, 2), F32(1, 1, 2)}, {F32(2, 0, 2), F32(2, 1, 2)}, {F32(4, 0, 2), F32(4, 1, 2)}},
     {{F32(1, 0, 0), F32(1, 1, 0)}, {F32(2, 0, 0), F32(2, 1, 0)}, {F32(4, 0, 0), F32(4, 1, 0)}},
+    {{F32M(1, 0), F32M(1, 1)}, {F32M(2, 0), F32M(2, 1)}, {F32M(4, 0), F32M(4, 1)}},
 };
 
 const void* bh_force_kernel(int precision, int pidx, int law)
 {
-    if (precision < 0 || precision > 3 || pidx < 0 || pidx > 2 || law < 0 || law > 1) return nullptr;
+    if (precision < 0 || precision > 4 || pidx < 0 || pidx > 2 || law < 0 || law > 1) return nullptr;
     return (const void*)g_force[precision][pidx][law];
 }

@@ -27,6 +27,11 @@ struct ForceArgs {
     double *ax, *ay;
     double Gc, eps2, r2cut;
     long long* cost_log;       // per item duration (wall clock ticks) or NULL
+    // tail split (FP32 kernels): the last n_tail queue positions are each run
+    // as `split` far-field parts; part (N x split x 2 doubles, by particle)
+    // holds their sums, added in part order by k_force_combine.  split == 1: off.
+    int split, n_tail;
+    double* part;
 };
 
 // placement
@@ -72,6 +77,9 @@ __global__ void k_interactions(const SysDev* sys, int S, const int32_t* counts,
 
 // force
 __global__ void k_clamp_items(int32_t* stats, int64_t max_items);
+__global__ void k_force_combine(const Item* items, const int32_t* order, const int32_t* stats,
+                                int split, int n_tail, int pslots, const double* part, double* ax,
+                                double* ay);
 template <bool WRITE>
 __global__ void k_items(const SysDev* sys, int S, int64_t ncols, const int32_t* counts,
                         const int32_t* offsets, int pitem, int yspan, int32_t* col_items,

@@ -216,7 +216,7 @@ def reference_config_line(steps):
     from barneshut.engine import Engine
     n, G = 10_000_000, 120
     d = inputs.reference_normal(n, seed=1)
-    e = Engine([n], [G], tuning=dict(p=1, yspan=128, waves_per_cu=16))
+    e = Engine([n], [G], tuning=dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0))
     e.load(d)
     import torch
     e.step(1.0, 0.0, 0.0, "fp64", 0.005)
@@ -245,9 +245,13 @@ def main():
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--grid", type=int, default=GRID_DEFAULT)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--p", type=int, default=2, help="FP32 path: particles per lane (1, 2, 4)")
-    ap.add_argument("--yspan", type=int, default=128, help="FP32 path: max rows per work item")
-    ap.add_argument("--waves-per-cu", type=int, default=16)
+    ap.add_argument("--p", type=int, default=4, help="FP32 path: particles per lane (1, 2, 4)")
+    ap.add_argument("--yspan", type=int, default=512, help="max rows per work item")
+    ap.add_argument("--waves-per-cu", type=int, default=32)
+    ap.add_argument("--tail-split", type=int, default=8,
+                    help="FP32 path: far-field slices per split item (1 = off)")
+    ap.add_argument("--tail-items", type=int, default=1 << 20,
+                    help="FP32 path: split the last this many queue items (>= items: all)")
     ap.add_argument("--p64", type=int, default=1, help="FP64 path: particles per lane")
     ap.add_argument("--autotune", action="store_true",
                     help="pick each path's launch shape with barneshut.autotune first")
@@ -284,8 +288,10 @@ def main():
     log(f"generating N={args.n} clusters={N_CLUSTERS} seed={args.seed}")
     d = inputs.gaussian_mixture(args.n, N_CLUSTERS, SIGMA, seed=args.seed)
     G = args.grid
-    tuning = dict(p=args.p, yspan=args.yspan, waves_per_cu=args.waves_per_cu)
-    tuning64 = dict(p=args.p64, yspan=args.yspan, waves_per_cu=args.waves_per_cu)
+    tuning = dict(p=args.p, yspan=args.yspan, waves_per_cu=args.waves_per_cu,
+                  tail_split=args.tail_split, tail_items=args.tail_items)
+    tuning64 = dict(p=args.p64, yspan=args.yspan, waves_per_cu=args.waves_per_cu, tail_split=1,
+                    tail_items=0)
 
     def measure(precision, tun, order_model=None):
         eng = Engine([args.n], [G], tuning=tun)

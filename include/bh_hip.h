@@ -44,7 +44,9 @@ enum {
     BH_PREC_FP64 = 0,            /* plain FP64 forces */
     BH_PREC_FP32_KAHAN = 1,      /* FP32 cell-relative, Kahan-summed 64-term tiles */
     BH_PREC_FP32_KAHAN_TERM = 2, /* FP32 cell-relative, Kahan per term */
-    BH_PREC_FP32_PLAIN = 3       /* FP32 cell-relative, uncompensated */
+    BH_PREC_FP32_PLAIN = 3,      /* FP32 cell-relative, uncompensated */
+    BH_PREC_FP32_MFMA = 4        /* FP32 cell-relative, far-field sums on the matrix cores,
+                                    Kahan-summed 64-term tiles */
 };
 
 /* SoA particle buffers (device), length = total particles of all systems. */
@@ -64,11 +66,18 @@ typedef struct bh_law {
 
 /* Launch shape of the force kernel. p: particles per lane (1, 2 or 4);
  * yspan: max rows per work item (>= 1); waves_per_cu: persistent waves per
- * compute unit (>= 1). */
+ * compute unit (>= 1); tail_split: the FP32 kernels run each of the last
+ * tail_items queue entries as tail_split far-field slices (1 = off, <= 8;
+ * tail_items >= the item count splits every item), so the queue drains in
+ * finer pieces; the slices' sums are added in a fixed order by a second
+ * kernel (scratch: particles x tail_split x 16 bytes).
+ * Default: p 4, yspan 512, waves_per_cu 32, tail_split 8, tail_items 2^20. */
 typedef struct bh_tuning {
     int32_t p;
     int32_t yspan;
     int32_t waves_per_cu;
+    int32_t tail_split;
+    int32_t tail_items;
 } bh_tuning;
 
 /* What bh_export copies out. */

@@ -160,6 +160,7 @@ def test_forces_fp64(law):
 
 
 @pytest.mark.parametrize("prec,tol_norm,tol_p99", [("fp32", 1e-5, 1e-5),
+                                                   ("fp32_mfma", 1e-5, 1e-5),
                                                    ("fp32_kahan_term", 1e-5, 1e-5),
                                                    ("fp32_plain", 1e-4, 1e-4)])
 def test_forces_fp32(prec, tol_norm, tol_p99):
@@ -332,4 +333,27 @@ def test_interaction_counts():
     near, far = eng.interactions()[-1]
     ref = orc.interactions(G, eng.counts().astype(np.int64))
     assert (near, far) == ref
+    eng.close()
+
+
+@pytest.mark.parametrize("split,tail", [(2, 40), (4, 1000), (8, 7)])
+def test_fp32_tail_split_matches_oracle(split, tail):
+    """The last `tail` queue items run as `split` far-field parts added by a
+    second kernel: same accuracy as the unsplit pass, deterministic."""
+    d = inputs.gaussian_mixture(60000, 16, (0.005, 0.05), seed=31)
+    G = 128
+    tun = dict(p=2, yspan=64, waves_per_cu=8, tail_split=split, tail_items=tail)
+    eng = _forces_setup(d, G, tun)
+    eng.evaluate(1.0, 1e-8, 0.0, "fp32")
+    eng.check()
+    first = eng.fetch()
+    h, com, rx, ry = _oracle_forces(eng, G, 1.0, 1e-8, 0.0)
+    norm, per = _rel(first["ax"], first["ay"], rx, ry)
+    assert norm <= 1e-5 and float(np.quantile(per, 0.99)) <= 1e-5
+    for k in ("ax", "ay"):
+        eng.parts[k].zero_()
+    eng.evaluate(1.0, 1e-8, 0.0, "fp32")
+    again = eng.fetch()
+    np.testing.assert_array_equal(again["ax"], first["ax"])
+    np.testing.assert_array_equal(again["ay"], first["ay"])
     eng.close()

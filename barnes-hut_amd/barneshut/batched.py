@@ -16,13 +16,19 @@ import numpy as np
 from .engine import Engine
 
 
+# Many small systems: 128-particle items and no far-field slicing (most
+# systems' COM lists are shorter than one slice would be worth).
+BATCHED_TUNING = dict(p=2, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0)
+
+
 class BatchedSystems:
 
     def __init__(self, sizes, grid_dims, data, law=(1.0, 0.0, 0.0), precision="fp32", dt=0.0,
                  tuning=None, device=0, model=None, graph=True):
         self.sizes = np.asarray(sizes, np.int64)
         self.grid_dims = np.asarray(grid_dims, np.int32)
-        self.eng = Engine(self.sizes, self.grid_dims, device=device, tuning=tuning)
+        self.eng = Engine(self.sizes, self.grid_dims, device=device,
+                          tuning=BATCHED_TUNING if tuning is None else tuning)
         self.eng.load(data)
         self.law = law
         self.precision = precision

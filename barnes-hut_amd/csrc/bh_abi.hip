@@ -239,7 +239,7 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     return ensure_part(c);
 }
 
-// Scratch of the split force pass: N x tail_split x 2 doubles.
+// Scratch of the split force pass: tail_split planes of N x 2 doubles.
 static int ensure_part(Ctx* c)
 {
     size_t need = (c->tune.tail_split > 1 && c->tune.tail_items > 0 && c->N > 0)
@@ -520,6 +520,7 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     fa.split = (fp32v && c->tune.tail_split > 1 && c->tune.tail_items > 0) ? c->tune.tail_split : 1;
     fa.n_tail = fa.split > 1 ? c->tune.tail_items : 0;
     fa.part = c->part;
+    fa.part_n = c->N;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
     void* args[] = {&fa};
     unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
@@ -528,7 +529,7 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
         const int pslots = 64 * c->tune.p;
         hipLaunchKernelGGL(k_force_combine, dim3(cdiv((int64_t)fa.n_tail * pslots, 256)), dim3(256),
                            0, st, (const Item*)c->items, order, (const int32_t*)c->stats, fa.split,
-                           fa.n_tail, pslots, (const double*)c->part, p->ax, p->ay);
+                           fa.n_tail, pslots, (const double*)c->part, fa.part_n, p->ax, p->ay);
     }
     LAUNCH_CHECK("evaluate");
     return BH_OK;

@@ -316,10 +316,23 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
 // term's separation carries a relative error of a few ulp.  Near-field
 // separations are formed in FP64 and rounded once.  KAHAN: 0 plain FP32
 // sums; 1 FP32 partial per 64-entry tile, tiles Kahan-summed; 2 Kahan per term.
+//
+// Far field without a cutoff (LAW 0) in mass-scaled form: with c = s / M per
+// COM entry (s a power of two per item),
+//     t = c (a - u),   M (a - u) / (|a - u|^2 + eps2) = s * t / (|t|^2 + c^2 eps2),
+// so the tile holds (c a_x, c a_y, -c, c^2 eps2) and each term costs two fma
+// for t, two for |t|^2, v_rcp_f32 and two fma into the sums: one VALU
+// instruction fewer than forming M * w.  Scaling by the power of two s is
+// exact, so s only keeps c^2 |a - u|^2 inside the FP32 range (s = 2^e with
+// 2^e <= M of the item's own cell < 2^(e+1)); near-field terms are divided by s
+// (exactly) to share the sums, and the sums are multiplied by s at the end.
+// Zero-mass entries and the padding past the list hold (1e30, 1e30, 0, 0):
+// |t|^2 overflows to inf, v_rcp_f32 gives 0 and the term adds exactly 0.
 template <int P, int LAW, int KAHAN>
 __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
 {
     __shared__ float4 ftile[BH_WAVE];
+    __shared__ int32_t ktile[BH_WAVE];         // kx << 16 | ky (LAW 0 masked tiles)
     __shared__ NearE ntile[BH_WAVE];
     const int lane = threadIdx.x;
     const int nitems = A.stats[ST_NITEMS];
@@ -331,10 +344,10 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
         const int q = next_item(A.stats, lane);
         if (q >= total) break;
         int pos = q, part = -1;
-        if (q >= nu) {
+        if (q >= nu) {   // slice-major: every tail item's slice 0, then slice 1, ...
             const int r = q - nu;
-            pos = nu + r / S;
-            part = r - (r / S) * S;
+            part = r / nt;
+            pos = nu + (r - part * nt);
         }
         const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
         const int itx = A.order ? uniform_int(A.order[pos]) : pos;
@@ -350,6 +363,15 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
         const double inv_step = 1.0 / step;
         const float eps2c = (float)(A.eps2 * inv_step * inv_step);
         const float r2c = (float)(A.r2cut * inv_step * inv_step);
+        // mass scale s (LAW 0): the power of two at or below the item's own cell mass
+        float msc = 1.0f;
+        if (LAW == 0) {
+            const int64_t own = coff + (int64_t)cx * G + it.y0;
+            const float m0 = fabsf(A.l32[A.neprefix[own]].M);
+            const float p2 = __int_as_float(__float_as_int(m0) & 0x7f800000);
+            if (p2 > 0.0f && p2 < 3.0e38f) msc = p2;
+        }
+        const float inv_msc = 1.0f / msc;
 
         double xi[P], yi[P];
         float ux[P], uy[P], Sx[P], Sy[P], Cx[P], Cy[P], tx[P], ty[P];
@@ -402,7 +424,7 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                 const int cnt = min(BH_WAVE, pb - base);
                 for (int jj = 0; jj < cnt; ++jj) {
                     const NearE f = ntile[jj];
-                    const float fm = (float)f.m;
+                    const float fm = (float)f.m * inv_msc;
 #pragma unroll
                     for (int k = 0; k < P; ++k) {
                         const bool keep = (f.idx != gi[k]) && (abs(f.row - cy[k]) <= 1);
@@ -463,11 +485,29 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
         for (int base = ne0; base < ne1; base += BH_WAVE) {
             {
                 const int kx = nxt.kxy >> 16, ky = nxt.kxy & 0xffff;
+                const float ex = (float)(kx - cx) + nxt.fx;
+                const float ey = (float)(ky - oy) + nxt.fy;
                 float4 v;
-                v.x = (float)(kx - cx) + nxt.fx;
-                v.y = (float)(ky - oy) + nxt.fy;
-                v.z = nxt.M;
-                v.w = __int_as_float(nxt.kxy);
+                if (LAW == 0) {
+                    if (nxt.M != 0.0f) {
+                        const float c = msc / nxt.M;
+                        v.x = ex * c;
+                        v.y = ey * c;
+                        v.z = -c;
+                        v.w = (c * c) * eps2c;
+                    } else {
+                        v.x = 1e30f;
+                        v.y = 1e30f;
+                        v.z = 0.0f;
+                        v.w = 0.0f;
+                    }
+                    ktile[lane] = nxt.kxy;
+                } else {
+                    v.x = ex;
+                    v.y = ey;
+                    v.z = nxt.M;
+                    v.w = __int_as_float(nxt.kxy);
+                }
                 ftile[lane] = v;
             }
             __syncthreads();
@@ -485,8 +525,16 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                     const float4 f = ftile[j];
 #pragma unroll
                     for (int k = 0; k < P; ++k) {
-                        const float wx = f.x - ux[k], wy = f.y - uy[k];
-                        const float t = far_t32<LAW>(wx, wy, f.z, eps2c, r2c);
+                        float wx, wy, t;
+                        if (LAW == 0) {
+                            wx = fmaf(ux[k], f.z, f.x);
+                            wy = fmaf(uy[k], f.z, f.y);
+                            t = __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, f.w)));
+                        } else {
+                            wx = f.x - ux[k];
+                            wy = f.y - uy[k];
+                            t = far_t32<LAW>(wx, wy, f.z, eps2c, r2c);
+                        }
                         if (KAHAN == 2) {
                             kahan_add(Sx[k], Cx[k], wx * t);
                             kahan_add(Sy[k], Cy[k], wy * t);
@@ -499,14 +547,22 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
             } else {
                 for (int j = 0; j < BH_WAVE; ++j) {
                     const float4 f = ftile[j];
-                    const int kxy = __float_as_int(f.w);
+                    const int kxy = LAW == 0 ? ktile[j] : __float_as_int(f.w);
                     const bool colnear = abs((kxy >> 16) - cx) <= 1;
                     const int ky = kxy & 0xffff;
 #pragma unroll
                     for (int k = 0; k < P; ++k) {
                         const bool keep = !(colnear && abs(ky - cy[k]) <= 1);
-                        const float wx = f.x - ux[k], wy = f.y - uy[k];
-                        float t = far_t32<LAW>(wx, wy, f.z, eps2c, r2c);
+                        float wx, wy, t;
+                        if (LAW == 0) {
+                            wx = fmaf(ux[k], f.z, f.x);
+                            wy = fmaf(uy[k], f.z, f.y);
+                            t = __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, f.w)));
+                        } else {
+                            wx = f.x - ux[k];
+                            wy = f.y - uy[k];
+                            t = far_t32<LAW>(wx, wy, f.z, eps2c, r2c);
+                        }
                         t = keep ? t : 0.0f;
                         if (KAHAN == 2) {
                             kahan_add(Sx[k], Cx[k], wx * t);
@@ -530,7 +586,7 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
             __syncthreads();
         }
 
-        const double scale = A.Gc * inv_step;
+        const double scale = A.Gc * inv_step * (double)msc;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             if (act[k]) {
@@ -546,7 +602,7 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                     A.ax[gi[k]] += scale * rx;
                     A.ay[gi[k]] += scale * ry;
                 } else {
-                    const int64_t o = ((int64_t)gi[k] * S + part) * 2;
+                    const int64_t o = ((int64_t)part * A.part_n + gi[k]) * 2;
                     A.part[o] = scale * rx;
                     A.part[o + 1] = scale * ry;
                 }
@@ -562,6 +618,7 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
                                                        const int32_t* __restrict__ stats,
                                                        int split, int n_tail, int pslots,
                                                        const double* __restrict__ part,
+                                                       int64_t part_n,
                                                        double* __restrict__ ax,
                                                        double* __restrict__ ay)
 {
@@ -579,7 +636,7 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
     const int64_t gi = (int64_t)it.p0 + slot;
     double sx = 0.0, sy = 0.0;
     for (int s = 0; s < split; ++s) {
-        const int64_t o = (gi * split + s) * 2;
+        const int64_t o = ((int64_t)s * part_n + gi) * 2;
         sx += part[o];
         sy += part[o + 1];
     }

@@ -28,10 +28,13 @@ struct ForceArgs {
     double Gc, eps2, r2cut;
     long long* cost_log;       // per item duration (wall clock ticks) or NULL
     // tail split (FP32 kernels): the last n_tail queue positions are each run
-    // as `split` far-field parts; part (N x split x 2 doubles, by particle)
-    // holds their sums, added in part order by k_force_combine.  split == 1: off.
+    // as `split` far-field parts, issued slice-major so that resident waves
+    // share one slice of the COM list in L2; part holds their sums, one plane
+    // of part_n particles x 2 doubles per slice (element ((s * part_n) + i) * 2),
+    // added in slice order by k_force_combine.  split == 1: off.
     int split, n_tail;
     double* part;
+    int64_t part_n;
 };
 
 // placement
@@ -78,8 +81,8 @@ __global__ void k_interactions(const SysDev* sys, int S, const int32_t* counts,
 // force
 __global__ void k_clamp_items(int32_t* stats, int64_t max_items);
 __global__ void k_force_combine(const Item* items, const int32_t* order, const int32_t* stats,
-                                int split, int n_tail, int pslots, const double* part, double* ax,
-                                double* ay);
+                                int split, int n_tail, int pslots, const double* part,
+                                int64_t part_n, double* ax, double* ay);
 template <bool WRITE>
 __global__ void k_items(const SysDev* sys, int S, int64_t ncols, const int32_t* counts,
                         const int32_t* offsets, int pitem, int yspan, int32_t* col_items,

@@ -179,6 +179,41 @@ def test_forces_fp32(prec, tol_norm, tol_p99):
     eng.close()
 
 
+def test_fp32_mass_scale_exact_and_zero_mass_cells():
+    """The FP32 far field runs in mass-scaled form with a power-of-two scale
+    per item: scaling every mass by 2^k and G by 2^-k must give bit-identical
+    accelerations (also where 1/M alone would leave the FP32 range), and cells
+    of zero total mass add nothing, as in the oracle."""
+    base = inputs.gaussian_mixture(30000, 12, (0.005, 0.05), seed=33)
+    G = 64
+    out = {}
+    for k in (0, 100, -100):
+        d = {f: np.array(v, copy=True) for f, v in base.items()}
+        d["mass"] = d["mass"] * 2.0 ** k
+        eng = _forces_setup(d, G)
+        eng.evaluate(2.0 ** -k, 1e-8, 0.0, "fp32")
+        eng.check()
+        out[k] = eng.fetch()
+        eng.close()
+        print(f"[mass scale 2^{k}] max|a|={np.abs(out[k]['ax']).max():.3e}", flush=True)
+    for k in (100, -100):
+        np.testing.assert_array_equal(out[k]["ax"], out[0]["ax"])
+        np.testing.assert_array_equal(out[k]["ay"], out[0]["ay"])
+    d = {f: np.array(v, copy=True) for f, v in base.items()}
+    d["mass"][d["px"] < np.quantile(d["px"], 0.3)] = 0.0
+    eng = _forces_setup(d, G)
+    eng.evaluate(1.0, 1e-8, 0.0, "fp32")
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, 1.0, 1e-8, 0.0)
+    assert (com[:, 2] == 0).sum() > 0 and (eng.counts() > 0).sum() > (com[:, 2] != 0).sum()
+    g = eng.fetch()
+    assert np.isfinite(g["ax"]).all() and np.isfinite(g["ay"]).all()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    print(f"[zero-mass cells] normwise={norm:.3e}", flush=True)
+    assert norm <= 1e-5
+    eng.close()
+
+
 @pytest.mark.parametrize("G", [1, 2])
 def test_small_grid_is_direct_sum(G):
     """G <= 2: every cell is a neighbour, so the method is the direct sum."""

@@ -131,6 +131,12 @@ class Engine:
         n = int(st[hiplib.ST_NITEMS])
         return self.ctx.export_host(hiplib.EXPORT_ITEMS, 8 * n, np.int32).reshape(-1, 8)
 
+    def energy(self, G=1.0, eps2=0.0):
+        """[(U_s, K_s)] per system for the current state: potential of the law
+        (direct FP64 sum, O(n^2): a diagnostic for small and medium systems)
+        and kinetic energy."""
+        return self.ctx.energy(self.parts, G, eps2)
+
     def interactions(self):
         """(near, far) per system and total, exact, from the current counts."""
         out = self.ctx.interactions()

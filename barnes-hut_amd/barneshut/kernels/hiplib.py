@@ -33,7 +33,7 @@ EXPORTED = (
     "bh_sort_particles", "bh_summarize", "bh_build_items", "bh_evaluate_boxes",
     "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
-    "bh_order_by_cost", "bh_learned_order", "bh_query",
+    "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy",
 )
 
 
@@ -106,6 +106,7 @@ def load(path: str = LIB_PATH):
         "bh_order_by_cost": (ctypes.c_int, [vp, vp, vp, vp]),
         "bh_learned_order": (ctypes.c_int, [vp, vp, vp, vp]),
         "bh_query": (ctypes.c_int, [vp, i32, P(i64)]),
+        "bh_energy": (ctypes.c_int, [vp, P(BhParticles), dbl, dbl, P(dbl), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -287,6 +288,15 @@ class Context:
         check(self.L.bh_interactions(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                      stream_handle(stream)))
         return out
+
+    def energy(self, parts, G=1.0, eps2=0.0, stream=None):
+        """[(U_s, K_s)] per system: direct FP64 potential of the law and kinetic energy."""
+        p = particles_struct(parts)
+        out = np.zeros(2 * self.n_sys, np.float64)
+        check(self.L.bh_energy(self.h, ctypes.byref(p), float(G), float(eps2),
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                               stream_handle(stream)))
+        return out.reshape(-1, 2)
 
     def interactions_async(self, dev_out, stream=None):
         """dev_out: device int64 tensor [2*S+2] (near counts include self pairs)."""

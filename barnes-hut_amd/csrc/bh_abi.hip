@@ -78,6 +78,7 @@ struct bh_ctx_s {
     bh_tuning tune = {4, 512, 32, 8, 1 << 20};
     double* part = nullptr;
     size_t part_cap = 0;
+    double* energy_buf = nullptr;   // 2 x N, allocated by the first bh_energy
     hipStream_t cap_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -170,6 +171,7 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipDeviceSynchronize());
     free_all(c);
+    c->energy_buf = nullptr;
     c->sys_h.assign(nsys, SysDev{});
     int64_t N = 0, C = 0, cols = 0, maxN = 0;
     for (int s = 0; s < nsys; ++s) {
@@ -734,6 +736,46 @@ int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream)
                        (const SysDev*)c->sys, c->S, (const int32_t*)c->counts,
                        (const int32_t*)c->neprefix, c->C, (unsigned long long*)dev_out);
     LAUNCH_CHECK("interactions");
+    return BH_OK;
+}
+
+int bh_energy(void* ctx, const bh_particles* p, double G, double eps2, double* out, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!out) return fail(BH_EINVAL, "out is NULL");
+    if (int e = check_parts(p, "energy")) return e;
+    if (!c->energy_buf)
+        if (int e = dalloc(c, &c->energy_buf, 2 * c->N)) return e;
+    hipStream_t st = (hipStream_t)stream;
+    double* u = c->energy_buf;
+    double* k = c->energy_buf + c->N;
+    for (int s = 0; s < c->S; ++s) {
+        const SysDev& d = c->sys_h[s];
+        if (d.n == 0) continue;
+        hipLaunchKernelGGL(k_energy, dim3((unsigned)cdiv(d.n, 256)), dim3(256), 0, st, p->px, p->py,
+                           p->mass, p->vx, p->vy, d.p_off, d.n, eps2, u, k);
+    }
+    LAUNCH_CHECK("energy");
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<double> h(2 * c->N);
+    HIP_TRY(hipMemcpy(h.data(), c->energy_buf, sizeof(double) * 2 * c->N, hipMemcpyDeviceToHost));
+    for (int s = 0; s < c->S; ++s) {
+        const SysDev& d = c->sys_h[s];
+        // Neumaier sums in row order
+        double su = 0.0, cu = 0.0, sk = 0.0, ck = 0.0;
+        for (int64_t i = d.p_off; i < d.p_off + d.n; ++i) {
+            double v = h[i], t = su + v;
+            cu += (std::fabs(su) >= std::fabs(v)) ? (su - t) + v : (v - t) + su;
+            su = t;
+            v = h[c->N + i];
+            t = sk + v;
+            ck += (std::fabs(sk) >= std::fabs(v)) ? (sk - t) + v : (v - t) + sk;
+            sk = t;
+        }
+        out[2 * s] = 0.25 * G * (su + cu);
+        out[2 * s + 1] = 0.5 * (sk + ck);
+    }
     return BH_OK;
 }
 

@@ -77,6 +77,9 @@ __global__ void k_tick(double* x, double* y, double* vx, double* vy, double* ax,
 __global__ void k_interactions(const SysDev* sys, int S, const int32_t* counts,
                                const int32_t* neprefix, int64_t C,
                                unsigned long long* out /* 2*S + 2 */);
+__global__ void k_energy(const double* x, const double* y, const double* m, const double* vx,
+                         const double* vy, int64_t p0, int64_t n, double eps2, double* u,
+                         double* k);
 
 // force
 __global__ void k_clamp_items(int32_t* stats, int64_t max_items);

@@ -148,3 +148,45 @@ __global__ __launch_bounds__(256) void k_interactions(const SysDev* __restrict__
         }
     }
 }
+
+// Energy diagnostic (direct sum, FP64): for each particle i of one system's
+// rows [p0, p0 + n), u[i] = m_i * sum_{j != i} m_j * log(r_ij^2 + eps2) and
+// k[i] = m_i * (vx_i^2 + vy_i^2).  The host adds them in row order:
+// U = G/4 * sum u (each pair twice, log r = log(r^2)/2), K = 1/2 * sum k.
+// Sources are staged through LDS 256 at a time; the loop runs over the
+// system's real rows only.
+__global__ __launch_bounds__(256) void k_energy(const double* __restrict__ x,
+                                                const double* __restrict__ y,
+                                                const double* __restrict__ m,
+                                                const double* __restrict__ vx,
+                                                const double* __restrict__ vy, int64_t p0,
+                                                int64_t n, double eps2, double* __restrict__ u,
+                                                double* __restrict__ k)
+{
+    __shared__ double sx[256], sy[256], sm[256];
+    const int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = li < n;
+    const int64_t i = p0 + (act ? li : 0);
+    const double xi = x[i], yi = y[i];
+    double acc = 0.0;
+    for (int64_t base = 0; base < n; base += 256) {
+        const int64_t j = base + threadIdx.x;
+        __syncthreads();
+        if (j < n) {
+            sx[threadIdx.x] = x[p0 + j];
+            sy[threadIdx.x] = y[p0 + j];
+            sm[threadIdx.x] = m[p0 + j];
+        }
+        __syncthreads();
+        const int cnt = (int)min((int64_t)256, n - base);
+        for (int jj = 0; jj < cnt; ++jj) {
+            if (base + jj == li) continue;
+            const double dx = sx[jj] - xi, dy = sy[jj] - yi;
+            acc += sm[jj] * log(dx * dx + dy * dy + eps2);
+        }
+    }
+    if (act) {
+        u[i] = m[i] * acc;
+        k[i] = m[i] * (vx[i] * vx[i] + vy[i] * vy[i]);
+    }
+}

@@ -155,6 +155,14 @@ int bh_interactions(void* ctx, int64_t* out, void* stream);
  * sum_c n_c * (particles in the 3x3 block) (self pairs included: subtract
  * each system's N for the near count) and the far counts. */
 int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream);
+/* Energy diagnostic; the reference reports no energy, so there is no
+ * counterpart (it serves the energy-drift report of long runs).  Per system s:
+ * out[2s] = U_s = G * sum_{i<j} m_i m_j * log(r_ij^2 + eps2) / 2, the potential
+ * of the law a_i = G sum_j m_j (x_j - x_i) / (r^2 + eps2), summed directly in
+ * FP64 (O(n_s^2) work: a diagnostic for n_s up to a few million), and
+ * out[2s+1] = K_s = sum m v^2 / 2.  Rows of p in the context's row order.
+ * Synchronises. */
+int bh_energy(void* ctx, const bh_particles* p, double G, double eps2, double* out, void* stream);
 
 /* Learned cost model (MLP 8 -> 32 -> 32 -> 1, FP32, ReLU).
  * params: device float[BH_MLP_NPARAM] = W1[32x8], b1[32], W2[32x32], b2[32],

@@ -295,6 +295,38 @@ void bho_direct(int64_t n, const double *x, const double *y, const double *m,
     }
 }
 
+/* Energy of a state under the law above (no reference counterpart: python-bh
+ * reports no energy; this pins the energy diagnostic of the HIP library).
+ * U = Gc * sum_{i<j} m_i m_j * log(r_ij^2 + eps2) / 2  (the potential whose
+ * negative gradient is the eps2-softened law, r2_cut ignored), K = sum m v^2 / 2.
+ * Compensated sums; the pair loop visits each unordered pair once. */
+void bho_energy(int64_t n, const double *x, const double *y, const double *m,
+                const double *vx, const double *vy, double Gc, double eps2, int nthreads,
+                double *out2)
+{
+    double *row = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        csum u = {0, 0};
+        for (int64_t j = i + 1; j < n; ++j) {
+            double wx = x[j] - x[i], wy = y[j] - y[i];
+            cs_add(&u, m[j] * log(wx * wx + wy * wy + eps2));
+        }
+        row[i] = m[i] * cs_val(&u);
+    }
+    csum U = {0, 0}, K = {0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        cs_add(&U, row[i]);
+        cs_add(&K, m[i] * (vx[i] * vx[i] + vy[i] * vy[i]));
+    }
+    free(row);
+    out2[0] = 0.5 * Gc * cs_val(&U);
+    out2[1] = 0.5 * cs_val(&K);
+}
+
 /* Symplectic-Euler tick, gpu/grid.py:159-174 (g_tick_particles):
  * v += a*dt; a = 0; x += v*dt. Separate multiply and add (no FMA). */
 void bho_tick(int64_t n, double *x, double *y, double *vx, double *vy, double *ax, double *ay,

@@ -65,6 +65,8 @@ def lib():
                                  ctypes.c_double, _i64p, ctypes.c_int64, ctypes.c_int, _dp, _dp]
         L.bho_tick.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double]
         L.bho_interactions.argtypes = [ctypes.c_int, _i64p, _i64p]
+        L.bho_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -181,6 +183,16 @@ def direct(x, y, m, Gc=1.0, eps2=0.0, r2_cut=0.0, targets=None, nthreads=0):
     lib().bho_direct(n, _d(x), _d(y), _d(m), float(Gc), float(eps2), float(r2_cut), tp, nt,
                      int(nthreads), _d(ax), _d(ay))
     return ax, ay
+
+
+def energy(x, y, m, vx, vy, Gc=1.0, eps2=0.0, nthreads=0):
+    """(U, K): U = Gc * sum_{i<j} m_i m_j log(r^2 + eps2) / 2, K = sum m v^2 / 2.
+    No reference counterpart (python-bh reports no energy)."""
+    x, y, m, vx, vy = (_f64c(a) for a in (x, y, m, vx, vy))
+    out = np.zeros(2)
+    lib().bho_energy(len(x), _d(x), _d(y), _d(m), _d(vx), _d(vy), float(Gc), float(eps2),
+                     int(nthreads), _d(out))
+    return float(out[0]), float(out[1])
 
 
 def tick(x, y, vx, vy, ax, ay, dt):

@@ -1,0 +1,192 @@
+"""GPU checks of the large clustered workload (N = 67,108,864, 4096 Gaussian
+clusters plus a 20 % uniform background, 4096 x 4096 grid) and of the energy
+diagnostic used for its drift report.
+
+The full-size case costs ~1e15 far-field interactions per step (about three
+GPU-minutes per FP32 step), so it runs only with BH_LARGE=1 and checks one
+step through properties that do not depend on size: placement keys bit-exact
+against the oracle, rows sorted under the reference's total order and a
+permutation of the input, COMs against the oracle, FP32 accelerations of a
+random sample of particles against the oracle's grid-method sum.  The energy
+drift over 20 leapfrog steps is measured on the same mixture scaled to
+N = 2^18 on a 256 x 256 grid (the same mean particles per cell), where the
+O(N^2) energy diagnostic is cheap.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from conftest import PKG, gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+from oracle import oracle as orc  # noqa: E402
+from barneshut import inputs  # noqa: E402
+
+GC, DT = 1e-6, 1e-3              # law and step of the 4M-particle workload
+SIGMA = (0.001, 0.03)
+MODEL = os.path.join(PKG, "costmodels", "clusters256.npz")
+
+
+def _engine(n, G, tuning=None):
+    from barneshut.engine import Engine
+    return Engine([n], [G], tuning=tuning)
+
+
+def test_energy_matches_oracle():
+    """bh_energy (direct FP64 on the GPU) against the oracle, one system and
+    two batched systems (pairs only within a system)."""
+    from barneshut.engine import Engine
+    a = inputs.gaussian_mixture(3000, 4, (0.01, 0.1), seed=61)
+    b = inputs.uniform(1700, seed=62)
+    for d in (a, b):
+        d["vx"] = np.random.default_rng(1).normal(0, 0.1, len(d["px"]))
+        d["vy"] = np.random.default_rng(2).normal(0, 0.1, len(d["px"]))
+    eng = Engine([3000, 1700], [16, 8])
+    both = {k: np.concatenate([a[k], b[k]]) for k in a}
+    eng.load(both)
+    got = eng.energy(1.7, 1e-6)
+    eng.close()
+    for s, d in enumerate((a, b)):
+        U, K = orc.energy(d["px"], d["py"], d["mass"], d["vx"], d["vy"], 1.7, 1e-6)
+        print(f"[energy s={s}] U={got[s, 0]:.15e} oracle {U:.15e}  K={got[s, 1]:.6e}", flush=True)
+        assert got[s, 0] == pytest.approx(U, rel=1e-12)
+        assert got[s, 1] == pytest.approx(K, rel=1e-13)
+
+
+def _trajectory(d, G, precision, steps, model=None):
+    import torch
+    eng = _engine(len(d["px"]), G)
+    eng.load(d)
+    order = None
+    if model is not None:
+        order = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
+    E = [float(eng.energy(GC, 0.0)[0].sum())]
+    for _ in range(steps):
+        eng.create_tree()
+        eng.summarize()
+        eng.ctx.build_items()
+        if model is not None:
+            eng.ctx.learned_order(model.params, order)
+        eng.ctx.force_pass(eng.parts, GC, 0.0, 0.0, precision, order)
+        eng.timestep(DT)
+        E.append(float(eng.energy(GC, 0.0)[0].sum()))
+    eng.check()
+    eng.close()
+    return np.array(E)
+
+
+def test_energy_drift_scaled_large_mix():
+    """20 steps of the large mixture scaled to 2^18 particles: the FP32
+    learned-order path (FP64 COM and update) follows the FP64 path's energy.
+    Measured on MI355X: drift -4.5e-4 for both paths, max |E32 - E64| / |E0|
+    = 1.9e-8."""
+    import torch
+    from barneshut import costmodel
+    n, G = 1 << 18, 256
+    d = inputs.gaussian_mixture(n, 16, SIGMA, seed=71, background=0.2)
+    model = costmodel.CostModel.load(MODEL, device=torch.device("cuda"))
+    e64 = _trajectory(d, G, "fp64", 20)
+    e32 = _trajectory(d, G, "fp32", 20, model)
+    d64 = (e64 - e64[0]) / abs(e64[0])
+    d32 = (e32 - e32[0]) / abs(e32[0])
+    gap = np.abs(e32 - e64) / abs(e64[0])
+    print(f"[drift] E0={e64[0]:.9e} fp64 drift={d64[-1]:.3e} fp32 drift={d32[-1]:.3e} "
+          f"max|E32-E64|/|E0|={gap.max():.3e}", flush=True)
+    print("[drift] fp64 " + " ".join(f"{v:.2e}" for v in d64), flush=True)
+    print("[drift] fp32 " + " ".join(f"{v:.2e}" for v in d32), flush=True)
+    assert np.isfinite(e64).all() and np.isfinite(e32).all()
+    assert e32[0] == e64[0]
+    # the integrator's own drift (symplectic Euler, no softening) stays small,
+    # and FP32 forces move the energy history by < 0.1 % of that drift
+    assert np.abs(d64).max() <= 5e-3
+    assert gap.max() <= 1e-3 * np.abs(d64).max()
+
+
+def _sorted_under_total_order(h, keys):
+    """Adjacent rows are non-decreasing in (cell, px, py, mass, vx, vy, ax, ay,
+    pid), the order of the reference's structured sort (sequential.py:37).
+    The accelerations were zero when the rows were sorted, so they tie and
+    drop out of the comparison."""
+    cols = [keys.astype(np.int64)] + [h[k] for k in ("px", "py", "mass", "vx", "vy")] \
+        + [h["pid"].astype(np.int64)]
+    ok = np.zeros(len(keys) - 1, bool)
+    tie = np.ones(len(keys) - 1, bool)
+    for c in cols:
+        a, b = c[:-1], c[1:]
+        ok |= tie & (a < b)
+        tie &= a == b
+    return bool(ok.all())
+
+
+@pytest.mark.skipif(os.environ.get("BH_LARGE") != "1",
+                    reason="67M-particle case: ~3 GPU-minutes per step; set BH_LARGE=1")
+def test_large_full_size_one_step():
+    import torch
+    from barneshut import costmodel
+    n, G = 67108864, 4096
+    t = time.time()
+    d = inputs.gaussian_mixture(n, 4096, SIGMA, seed=81, background=0.2)
+    print(f"[large] generated N={n} in {time.time() - t:.1f}s", flush=True)
+    eng = _engine(n, G)
+    eng.load(d)
+    model = costmodel.CostModel.load(MODEL, device=torch.device("cuda"))
+    order = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record()
+    eng.create_tree()
+    eng.summarize()
+    eng.ctx.build_items()
+    eng.ctx.learned_order(model.params, order)
+    ev[1].record()
+    eng.ctx.force_pass(eng.parts, GC, 0.0, 0.0, "fp32", order)
+    ev[2].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    eng.check()
+    inter = eng.interactions()[-1]
+    fms = ev[1].elapsed_time(ev[2])
+    tflops = 11.0 * float(inter.sum()) / (fms * 1e-3) / 1e12
+    print(f"[large] tree+summary+items+order {ev[0].elapsed_time(ev[1]):.1f} ms, "
+          f"force {fms:.1f} ms, near {inter[0]:.3e} far {inter[1]:.3e}, {tflops:.1f} TFLOP/s, "
+          f"{n / wall / 1e6:.3f} Mparticle-steps/s (one step, no tick)", flush=True)
+
+    # placement: keys bit-exact, counts, sorted rows that permute the input
+    bb = orc.bbox(d["px"], d["py"], G)
+    assert tuple(eng.bbox()[0][[0, 1, 2, 4]]) == (bb[0], bb[1], bb[2], bb[4])
+    _, _, key = orc.place(d["px"], d["py"], bb[0], bb[1], bb[4], G)
+    np.testing.assert_array_equal(eng.keys_in(), key)
+    cnt = np.bincount(key, minlength=G * G)
+    np.testing.assert_array_equal(eng.counts(), cnt)
+    h = eng.fetch()
+    ks = eng.keys()
+    pid = h["pid"].astype(np.int64)
+    assert np.array_equal(np.sort(pid), np.arange(n))
+    for k in ("px", "py", "mass"):
+        assert np.array_equal(h[k], d[k][pid])
+    assert _sorted_under_total_order(h, ks)
+    print("[large] placement checks passed", flush=True)
+
+    # COMs against the oracle
+    off = eng.offsets().astype(np.int64)
+    ref = orc.summarize(off, h["px"], h["py"], h["mass"])
+    com = eng.com()
+    np.testing.assert_allclose(com[:, 2], ref[:, 2], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(com[:, :2], ref[:, :2], rtol=1e-12, atol=1e-15)
+
+    # FP32 accelerations of a random sample against the oracle
+    idx = np.sort(np.random.default_rng(5).choice(n, 256, replace=False))
+    t = time.time()
+    rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, 0.0, 0.0, targets=idx)
+    num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
+    den = np.hypot(rx, ry)
+    norm = float(np.sqrt((num ** 2).sum() / (den ** 2).sum()))
+    print(f"[large] oracle sample ({time.time() - t:.1f}s): normwise {norm:.3e}, "
+          f"p99 {np.quantile(num / den, 0.99):.3e}, max {(num / den).max():.3e}", flush=True)
+    assert norm <= 1e-5
+    eng.close()

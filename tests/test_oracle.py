@@ -13,6 +13,8 @@ Each phase is checked against something that does not restate its formula:
   * tick             -> Python float arithmetic (IEEE, no FMA)
   * interactions     -> brute-force pair enumeration
   * MLP              -> finite differences, hand-computed cases
+  * energy           -> two-body closed form; its gradient against the direct
+                        force of the same law (finite differences)
 """
 import json
 import os
@@ -275,3 +277,32 @@ def test_adam_first_step():
     q, m, v = omlp.adam(p, g, np.zeros(3), np.zeros(3), 1, 0.1)
     # first step: m_hat = g, v_hat = g^2 -> update -lr * sign(g)
     np.testing.assert_allclose(q, [0.9, -1.9, 3.0], atol=1e-7)
+
+
+def test_energy_two_body_closed_form():
+    x, y = np.array([0.1, 0.4]), np.array([0.2, 0.6])
+    m, vx, vy = np.array([1.5, 2.0]), np.array([0.3, -0.1]), np.array([0.0, 0.2])
+    U, K = orc.energy(x, y, m, vx, vy, Gc=0.7, eps2=0.0)
+    r = np.hypot(0.3, 0.4)
+    assert U == pytest.approx(0.7 * 1.5 * 2.0 * np.log(r), rel=1e-14)
+    assert K == pytest.approx(0.5 * (1.5 * 0.09 + 2.0 * (0.01 + 0.04)), rel=1e-14)
+
+
+def test_energy_gradient_is_the_force_law():
+    """-dU/dx_i = m_i a_i with a from the direct sum of the same softened law
+    (central differences): pins the potential against the force."""
+    rng = np.random.default_rng(7)
+    n = 7
+    x, y, m = rng.random(n), rng.random(n), rng.uniform(1, 2, n)
+    z = np.zeros(n)
+    Gc, eps2, h = 1.3, 1e-3, 1e-6
+    ax, ay = orc.direct(x, y, m, Gc, eps2, 0.0)
+    for i in range(n):
+        for arr, acc in ((x, ax), (y, ay)):
+            arr[i] += h
+            up = orc.energy(x, y, m, z, z, Gc, eps2)[0]
+            arr[i] -= 2 * h
+            dn = orc.energy(x, y, m, z, z, Gc, eps2)[0]
+            arr[i] += h
+            grad = (up - dn) / (2 * h)
+            assert -grad == pytest.approx(m[i] * acc[i], rel=1e-6, abs=1e-8)

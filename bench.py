@@ -50,8 +50,13 @@ FLOP_PER_INTERACTION = 11
 # HBM bytes per force-kernel launch on this workload: rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled by the readings of a
 # 1 GiB device copy (FETCH x2.0, WRITE x1.0), measured on the first two
-# steps of this workload with the default launch shape.
-TRAFFIC = 2.02e9
+# steps of this workload with the default launch shape
+# (profiles/force32_gauss64_n4M_g1024_hbm_traffic.md).
+TRAFFIC = 1.80e9
+# Issue-limited ceiling of the FP32 far-field term: 6 VALU instructions per
+# interaction (v_rcp_f32 co-issued), no packed FP32, 256 CUs x 4 SIMDs x 16
+# lanes x 2.4 GHz = 39.3e12 lane-instructions/s -> 6.55e12 interactions/s.
+ISSUE_CEILING_TFLOPS = 256 * 4 * 16 * 2.4e9 / 6 * FLOP_PER_INTERACTION / 1e12
 
 
 def log(msg):
@@ -371,7 +376,9 @@ def main():
                      "unit": "TFLOP/s", "frac": res["achieved"] / FP32_PEAK_TFLOPS,
                      "traffic": TRAFFIC, "kernel": "k_force32<P,0,1>",
                      "force_kernel_ms": res["force_kernel_ms"],
-                     "flop_per_launch": res["flop_per_launch"]},
+                     "flop_per_launch": res["flop_per_launch"],
+                     "issue_ceiling": ISSUE_CEILING_TFLOPS,
+                     "frac_of_issue_ceiling": res["achieved"] / ISSUE_CEILING_TFLOPS},
     }
     if res_static is not None:
         out["fp32_static"] = {"value": res_static["value"],

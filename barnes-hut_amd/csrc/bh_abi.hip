@@ -732,7 +732,7 @@ int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream)
     hipStream_t st = (hipStream_t)stream;
     const int64_t n2 = 2 * (int64_t)c->S + 2;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, (int32_t*)dev_out, 2 * n2, 0);
-    hipLaunchKernelGGL(k_interactions, dim3(grid_for(c->C, 256, 8192)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_interactions, dim3(grid_for(c->C, 256, 1024)), dim3(256), 0, st,
                        (const SysDev*)c->sys, c->S, (const int32_t*)c->counts,
                        (const int32_t*)c->neprefix, c->C, (unsigned long long*)dev_out);
     LAUNCH_CHECK("interactions");
@@ -787,7 +787,7 @@ int bh_interactions(void* ctx, int64_t* out, void* stream)
     hipStream_t st = (hipStream_t)stream;
     const int64_t n2 = 2 * (int64_t)c->S + 2;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, st, (int32_t*)c->inter, 2 * n2, 0);
-    hipLaunchKernelGGL(k_interactions, dim3(grid_for(c->C, 256, 8192)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_interactions, dim3(grid_for(c->C, 256, 1024)), dim3(256), 0, st,
                        (const SysDev*)c->sys, c->S, (const int32_t*)c->counts,
                        (const int32_t*)c->neprefix, c->C, c->inter);
     LAUNCH_CHECK("interactions");

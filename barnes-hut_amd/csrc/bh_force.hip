@@ -331,8 +331,9 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
 template <int P, int LAW, int KAHAN>
 __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
 {
-    __shared__ float4 ftile[BH_WAVE];
-    __shared__ int32_t ktile[BH_WAVE];         // kx << 16 | ky (LAW 0 masked tiles)
+    constexpr int FT = 2 * BH_WAVE;            // far-field tile: two entries staged per lane
+    __shared__ float4 ftile[FT];
+    __shared__ int32_t ktile[FT];              // kx << 16 | ky (LAW 0 masked tiles)
     __shared__ NearE ntile[BH_WAVE];
     const int lane = threadIdx.x;
     const int nitems = A.stats[ST_NITEMS];
@@ -476,21 +477,24 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                 mhi[t] = 0;
             }
         }
-        Com32 nxt;
-        {
-            const int e = ne0 + lane;
-            if (e < ne1) nxt = A.l32[e];
-            else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+        Com32 nxt[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = ne0 + lane + h * BH_WAVE;
+            if (e < ne1) nxt[h] = A.l32[e];
+            else { nxt[h].fx = 1e18f; nxt[h].fy = 1e18f; nxt[h].M = 0.0f; nxt[h].kxy = 0x7fff7fff; }
         }
-        for (int base = ne0; base < ne1; base += BH_WAVE) {
-            {
-                const int kx = nxt.kxy >> 16, ky = nxt.kxy & 0xffff;
-                const float ex = (float)(kx - cx) + nxt.fx;
-                const float ey = (float)(ky - oy) + nxt.fy;
+        for (int base = ne0; base < ne1; base += FT) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const Com32 en = nxt[h];
+                const int kx = en.kxy >> 16, ky = en.kxy & 0xffff;
+                const float ex = (float)(kx - cx) + en.fx;
+                const float ey = (float)(ky - oy) + en.fy;
                 float4 v;
                 if (LAW == 0) {
-                    if (nxt.M != 0.0f) {
-                        const float c = msc / nxt.M;
+                    if (en.M != 0.0f) {
+                        const float c = msc / en.M;
                         v.x = ex * c;
                         v.y = ey * c;
                         v.z = -c;
@@ -501,27 +505,28 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                         v.z = 0.0f;
                         v.w = 0.0f;
                     }
-                    ktile[lane] = nxt.kxy;
+                    ktile[lane + h * BH_WAVE] = en.kxy;
                 } else {
                     v.x = ex;
                     v.y = ey;
-                    v.z = nxt.M;
-                    v.w = __int_as_float(nxt.kxy);
+                    v.z = en.M;
+                    v.w = __int_as_float(en.kxy);
                 }
-                ftile[lane] = v;
+                ftile[lane + h * BH_WAVE] = v;
             }
             __syncthreads();
-            {
-                const int e = base + BH_WAVE + lane;
-                if (e < ne1) nxt = A.l32[e];
-                else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int e = base + FT + lane + h * BH_WAVE;
+                if (e < ne1) nxt[h] = A.l32[e];
+                else { nxt[h].fx = 1e18f; nxt[h].fy = 1e18f; nxt[h].M = 0.0f; nxt[h].kxy = 0x7fff7fff; }
             }
-            const int b1 = base + BH_WAVE;
+            const int b1 = base + FT;
             const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||
                                 meets(base, b1, mlo[2], mhi[2]);
             if (!masked) {
 #pragma unroll 8
-                for (int j = 0; j < BH_WAVE; ++j) {
+                for (int j = 0; j < FT; ++j) {
                     const float4 f = ftile[j];
 #pragma unroll
                     for (int k = 0; k < P; ++k) {
@@ -545,7 +550,7 @@ __global__ __launch_bounds__(64) void k_force32(ForceArgs A)
                     }
                 }
             } else {
-                for (int j = 0; j < BH_WAVE; ++j) {
+                for (int j = 0; j < FT; ++j) {
                     const float4 f = ftile[j];
                     const int kxy = LAW == 0 ? ktile[j] : __float_as_int(f.w);
                     const bool colnear = abs((kxy >> 16) - cx) <= 1;

@@ -104,6 +104,22 @@ __global__ __launch_bounds__(256) void k_interactions(const SysDev* __restrict__
                                                       const int32_t* __restrict__ neprefix,
                                                       int64_t C, unsigned long long* out)
 {
+    // Each wave keeps running sums for the system its cells belong to and
+    // flushes them with atomics only when that system changes and at the end
+    // (cells of one system are contiguous), so the four counters see a few
+    // atomics per wave instead of one per 64 cells.
+    const int lane = threadIdx.x & 63;
+    unsigned long long acc_n = 0, acc_f = 0;
+    int acc_s = -1;
+    auto flush = [&]() {
+        if (lane == 0 && acc_s >= 0) {
+            atomicAdd(&out[2 * acc_s], acc_n);
+            atomicAdd(&out[2 * acc_s + 1], acc_f);
+            atomicAdd(&out[2 * S], acc_n);
+            atomicAdd(&out[2 * S + 1], acc_f);
+        }
+        acc_n = acc_f = 0;
+    };
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < C; c0 += stride) {
         int64_t c = c0 + threadIdx.x;
@@ -129,16 +145,17 @@ __global__ __launch_bounds__(256) void k_interactions(const SysDev* __restrict__
             nr = (unsigned long long)(nc * n9);
             fr = (unsigned long long)(nc * (K - k9));
         }
-        int s0 = __shfl(s, 0, BH_WAVE);
-        bool uni = __all(s == s0 || s < 0);
+        const int sv = wave_max(s);
+        const bool uni = __all(s == sv || s < 0);
         if (uni) {
-            int sv = wave_max(s);
-            unsigned long long a = wave_sum(nr), b = wave_sum(fr);
-            if ((threadIdx.x & 63) == 0 && sv >= 0) {
-                atomicAdd(&out[2 * sv], a);
-                atomicAdd(&out[2 * sv + 1], b);
-                atomicAdd(&out[2 * S], a);
-                atomicAdd(&out[2 * S + 1], b);
+            if (sv >= 0) {
+                const unsigned long long a = wave_sum(nr), b = wave_sum(fr);
+                if (sv != acc_s) {
+                    flush();
+                    acc_s = sv;
+                }
+                acc_n += a;
+                acc_f += b;
             }
         } else if (s >= 0) {
             atomicAdd(&out[2 * s], nr);
@@ -147,6 +164,7 @@ __global__ __launch_bounds__(256) void k_interactions(const SysDev* __restrict__
             atomicAdd(&out[2 * S + 1], fr);
         }
     }
+    flush();
 }
 
 // Energy diagnostic (direct sum, FP64): for each particle i of one system's

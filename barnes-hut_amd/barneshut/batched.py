@@ -16,9 +16,9 @@ import numpy as np
 from .engine import Engine
 
 
-# Many small systems: 128-particle items and no far-field slicing (most
+# Many small systems: 64-particle items and no far-field slicing (most
 # systems' COM lists are shorter than one slice would be worth).
-BATCHED_TUNING = dict(p=2, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0)
+BATCHED_TUNING = dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0)
 
 
 class BatchedSystems:

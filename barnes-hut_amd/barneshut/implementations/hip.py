@@ -44,7 +44,7 @@ class HipBarnesHut(BaseBarnesHut):
         self.schedule = Config.get("hip", "schedule", fallback="static")
         self.cost_model_path = Config.get("hip", "cost_model", fallback="")
         self.autotune = Config.get("hip", "autotune", fallback="False") == "True"
-        self.tuning = dict(p=int(Config.get("hip", "p", fallback="4")),
+        self.tuning = dict(p=int(Config.get("hip", "p", fallback="2")),
                            yspan=int(Config.get("hip", "yspan", fallback="512")),
                            waves_per_cu=int(Config.get("hip", "waves_per_cu", fallback="32")),
                            tail_split=int(Config.get("hip", "tail_split", fallback="8")),

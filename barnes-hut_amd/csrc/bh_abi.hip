@@ -75,7 +75,7 @@ struct bh_ctx_s {
     int32_t* cost_scratch = nullptr;
     float* feat = nullptr;
     float* pred = nullptr;
-    bh_tuning tune = {4, 512, 32, 8, 1 << 20};
+    bh_tuning tune = {2, 512, 32, 8, 1 << 20};
     double* part = nullptr;
     size_t part_cap = 0;
     double* energy_buf = nullptr;   // 2 x N, allocated by the first bh_energy

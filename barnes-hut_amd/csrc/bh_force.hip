@@ -328,8 +328,17 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
 // (exactly) to share the sums, and the sums are multiplied by s at the end.
 // Zero-mass entries and the padding past the list hold (1e30, 1e30, 0, 0):
 // |t|^2 overflows to inf, v_rcp_f32 gives 0 and the term adds exactly 0.
+// Minimum resident waves per SIMD the FP32 kernel is compiled for (bounds its
+// VGPRs so that more waves hide the reciprocal's latency): 5 for 4 particles
+// per lane, 7 for 2, 8 for 1 (one fewer with per-term Kahan sums, which hold
+// more registers).
+template <int P, int KAHAN>
+struct F32Waves {
+    static constexpr int value = (P >= 4 ? 5 : (P == 2 ? 7 : 8)) - (KAHAN == 2 ? 1 : 0);
+};
+
 template <int P, int LAW, int KAHAN>
-__global__ __launch_bounds__(64) void k_force32(ForceArgs A)
+__global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(ForceArgs A)
 {
     constexpr int FT = 2 * BH_WAVE;            // far-field tile: two entries staged per lane
     __shared__ float4 ftile[FT];

@@ -250,7 +250,7 @@ def main():
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--grid", type=int, default=GRID_DEFAULT)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--p", type=int, default=4, help="FP32 path: particles per lane (1, 2, 4)")
+    ap.add_argument("--p", type=int, default=2, help="FP32 path: particles per lane (1, 2, 4)")
     ap.add_argument("--yspan", type=int, default=512, help="max rows per work item")
     ap.add_argument("--waves-per-cu", type=int, default=32)
     ap.add_argument("--tail-split", type=int, default=8,

@@ -71,7 +71,7 @@ typedef struct bh_law {
  * tail_items >= the item count splits every item), so the queue drains in
  * finer pieces; the slices' sums are added in a fixed order by a second
  * kernel (scratch: particles x tail_split x 16 bytes).
- * Default: p 4, yspan 512, waves_per_cu 32, tail_split 8, tail_items 2^20. */
+ * Default: p 2, yspan 512, waves_per_cu 32, tail_split 8, tail_items 2^20. */
 typedef struct bh_tuning {
     int32_t p;
     int32_t yspan;

@@ -157,3 +157,44 @@ def test_collect_train_rank_on_logged_costs():
     assert np.isfinite(pred).all()
     print(f"rank quality on logged items: {costmodel.rank_quality(pred, c):.3f}", flush=True)
     e.close()
+
+
+def _force_ms(e, order, reps=3):
+    import torch
+    best = float("inf")
+    for _ in range(reps):
+        e.parts["ax"].zero_()
+        e.parts["ay"].zero_()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        e.ctx.force_pass(e.parts, 1e-6, 0.0, 0.0, "fp32", order)
+        b.record()
+        b.synchronize()
+        best = min(best, a.elapsed_time(b))
+    return best
+
+
+@pytest.mark.parametrize("split", [8, 1], ids=["sliced", "whole_items"])
+def test_trained_model_gain_over_static(split):
+    """What the shipped model (trained on 256 clusters, lognormal masses) gains
+    over the static queue order on that kind of input, with the default
+    far-field slicing and with whole items (where the order alone decides the
+    tail).  Reported; asserted only not to lose more than timing noise."""
+    import os
+    import torch
+    from conftest import PKG
+    d = inputs.gaussian_mixture(1 << 20, 256, (0.001, 0.03), seed=12, mass="lognormal")
+    tun = dict(p=2, yspan=512, waves_per_cu=32, tail_split=split,
+               tail_items=(1 << 20) if split > 1 else 0)
+    e = _engine(d, 512, tun)
+    model = costmodel.CostModel.load(os.path.join(PKG, "costmodels", "clusters256.npz"),
+                                     device=torch.device("cuda"))
+    order = torch.empty(e.ctx.query(4), dtype=torch.int32, device="cuda")
+    e.ctx.learned_order(model.params, order)
+    static = _force_ms(e, None)
+    learned = _force_ms(e, order)
+    print(f"[gain split={split}] static {static:.3f} ms, learned {learned:.3f} ms, "
+          f"gain {static / learned:.4f}", flush=True)
+    assert learned <= 1.03 * static
+    e.close()

@@ -37,13 +37,19 @@ __global__ void k_bbox_init(uint64_t* bb, int S)
     }
 }
 
+// Atomic min/max of the ordered keys; skipped when the stored value already
+// bounds ours (stored minima only fall and maxima only rise, so a stale read
+// can only cause an unneeded atomic, never a missed one).
 __device__ __forceinline__ void bbox_flush(uint64_t* bb, int s, double mnx, double mny, double mxx,
                                            double mxy)
 {
-    atomicMin((unsigned long long*)&bb[4 * s + 0], (unsigned long long)bh_ord_key(mnx));
-    atomicMin((unsigned long long*)&bb[4 * s + 1], (unsigned long long)bh_ord_key(mny));
-    atomicMax((unsigned long long*)&bb[4 * s + 2], (unsigned long long)bh_ord_key(mxx));
-    atomicMax((unsigned long long*)&bb[4 * s + 3], (unsigned long long)bh_ord_key(mxy));
+    unsigned long long* b = (unsigned long long*)&bb[4 * s];
+    const unsigned long long k0 = bh_ord_key(mnx), k1 = bh_ord_key(mny);
+    const unsigned long long k2 = bh_ord_key(mxx), k3 = bh_ord_key(mxy);
+    if (k0 < __atomic_load_n(&b[0], __ATOMIC_RELAXED)) atomicMin(&b[0], k0);
+    if (k1 < __atomic_load_n(&b[1], __ATOMIC_RELAXED)) atomicMin(&b[1], k1);
+    if (k2 > __atomic_load_n(&b[2], __ATOMIC_RELAXED)) atomicMax(&b[2], k2);
+    if (k3 > __atomic_load_n(&b[3], __ATOMIC_RELAXED)) atomicMax(&b[3], k3);
 }
 
 // Each block covers PLACE_CHUNK consecutive particles; min/max are exact so
@@ -442,6 +448,14 @@ __device__ __forceinline__ bool row_less(const SortFields& F, int a, int b)
     return tie_less(F, a, b);
 }
 
+// 64-bit key ordered as numpy orders doubles: -0.0 equals +0.0, every NaN is
+// equal to every other NaN and sorts after all numbers.
+__device__ __forceinline__ uint64_t sort_key(double d)
+{
+    if (isnan(d)) return ~0ull;
+    return bh_ord_key(d == 0.0 ? 0.0 : d);
+}
+
 // A "unit" is a whole cell when the cell holds <= SORT_W particles, else one
 // SORT_W-long chunk of it (chunks counted from the cell start).  Every unit
 // starts in exactly one aligned SORT_W window; the block owning that window
@@ -452,7 +466,7 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
                                                     const int32_t* __restrict__ offsets,
                                                     int32_t* __restrict__ idx_out)
 {
-    __shared__ double lpx[2 * SORT_W];
+    __shared__ uint64_t lkey[2 * SORT_W];
     __shared__ int32_t lidx[2 * SORT_W];
     const int64_t w0 = (int64_t)blockIdx.x * SORT_W;
     const int64_t wend = min((int64_t)N, w0 + 2 * SORT_W);
@@ -461,7 +475,7 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
         if (q < wend) {
             int e = idx_in[q];
             lidx[t] = e;
-            lpx[t] = F.px[e];
+            lkey[t] = sort_key(F.px[e]);
         }
     }
     __syncthreads();
@@ -481,12 +495,12 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
         }
         if (ua < w0 || ua >= w0 + SORT_W) continue;   // unit owned by another block
         int e = lidx[t];
-        double x = lpx[t];
+        const uint64_t x = lkey[t];
         int rank = 0;
         for (int j = (int)(ua - w0); j < (int)(ub - w0); ++j) {
-            if (j == t) continue;
-            const int c = dcmp(lpx[j], x);
-            rank += (c < 0) || (c == 0 && tie_less(F, lidx[j], e));
+            const uint64_t y = lkey[j];
+            if (y < x) ++rank;
+            else if (y == x && j != t && tie_less(F, lidx[j], e)) ++rank;
         }
         idx_out[ua + rank] = e;
     }

@@ -33,6 +33,8 @@ def _ties(n, seed=3):
     d["py"][j[:10]] = d["py"][j[10:]]
     d["px"][j[:10]] = d["px"][j[10:]]                     # identical positions
     d["mass"][j[:5]] = d["mass"][j[10:15]]                  # identical up to pid
+    z = np.nonzero(d["px"] == 0.0)[0]
+    d["px"][z[::2]] = -0.0                                  # -0.0 ties +0.0 (numpy order)
     return d
 
 

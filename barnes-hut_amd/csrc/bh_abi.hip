@@ -453,7 +453,7 @@ int bh_build_items(void* ctx, void* stream)
     CTX_READY(c);
     hipStream_t st = (hipStream_t)stream;
     const int pitem = 64 * c->tune.p;
-    unsigned g = (unsigned)cdiv(c->ncols, 64);
+    unsigned g = (unsigned)c->ncols;   // one wave per grid column
     hipLaunchKernelGGL(k_items<false>, dim3(g), dim3(64), 0, st, (const SysDev*)c->sys, c->S,
                        c->ncols, (const int32_t*)c->counts, (const int32_t*)c->offsets, pitem,
                        c->tune.yspan, c->col_items, c->items, c->max_items, c->stats);

@@ -24,20 +24,26 @@
 #include "bh_kernels.h"
 
 // ---------------------------------------------------------------- items
-// One thread per grid column (all systems).  The column's sorted particles
+// One wave per grid column (all systems).  The column's sorted particles
 // are cut, in order, into runs of pitem particles (a run may start or end
 // inside a cell); a run is closed early when its next particle lies yspan or
 // more rows below its first one, and at the end of the column.  Pass 1
 // (WRITE = false) counts items per column; pass 2 writes them at the scanned
 // column offsets.
 template <bool WRITE>
-__global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
-                        const int32_t* __restrict__ counts, const int32_t* __restrict__ offsets,
-                        int pitem, int yspan, int32_t* __restrict__ col_items,
-                        Item* __restrict__ items, int64_t max_items, int32_t* stats)
+__global__ __launch_bounds__(64) void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
+                                              const int32_t* __restrict__ counts,
+                                              const int32_t* __restrict__ offsets, int pitem,
+                                              int yspan, int32_t* __restrict__ col_items,
+                                              Item* __restrict__ items, int64_t max_items,
+                                              int32_t* stats)
 {
-    int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // One wave per column: the column's counts and offsets are read 64 rows at
+    // a time, and the greedy cut walks the non-empty rows with wave-uniform
+    // values (every lane holds the same state; lane 0 writes).
+    const int64_t col = blockIdx.x;
     if (col >= ncols) return;
+    const int lane = threadIdx.x;
     const int s = sys_of_col(sys, S, col);
     const SysDev& d = sys[s];
     const int G = d.G;
@@ -48,7 +54,7 @@ __global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
     bool open = false;
     int p0 = 0, np = 0, y0 = 0, y1 = 0;
     auto emit = [&]() {
-        if (WRITE) {
+        if (WRITE && lane == 0) {
             int64_t k = out + n_out;
             if (k < max_items) {
                 Item it;
@@ -68,28 +74,36 @@ __global__ void k_items(const SysDev* __restrict__ sys, int S, int64_t ncols,
         ++n_out;
         open = false;
     };
-    for (int yy = 0; yy < G; ++yy) {
-        int nc = counts[cb + yy];
-        if (nc == 0) continue;
-        int pc = offsets[cb + yy];
-        if (open && yy - y0 >= yspan) emit();
-        while (nc > 0) {
-            if (!open) {
-                open = true;
-                p0 = pc;
-                np = 0;
-                y0 = yy;
+    for (int yb = 0; yb < G; yb += BH_WAVE) {
+        const int yl = yb + lane;
+        const int ncl = yl < G ? counts[cb + yl] : 0;
+        const int pcl = yl < G ? offsets[cb + yl] : 0;
+        unsigned long long nz = __ballot(ncl > 0);
+        while (nz) {
+            const int l = __ffsll((long long)nz) - 1;
+            nz &= nz - 1;
+            const int yy = yb + l;
+            int nc = __shfl(ncl, l, BH_WAVE);
+            int pc = __shfl(pcl, l, BH_WAVE);
+            if (open && yy - y0 >= yspan) emit();
+            while (nc > 0) {
+                if (!open) {
+                    open = true;
+                    p0 = pc;
+                    np = 0;
+                    y0 = yy;
+                }
+                int take = min(nc, pitem - np);
+                np += take;
+                pc += take;
+                nc -= take;
+                y1 = yy;
+                if (np == pitem) emit();
             }
-            int take = min(nc, pitem - np);
-            np += take;
-            pc += take;
-            nc -= take;
-            y1 = yy;
-            if (np == pitem) emit();
         }
     }
     if (open) emit();
-    if (!WRITE) col_items[col] = n_out;
+    if (!WRITE && lane == 0) col_items[col] = n_out;
 }
 
 template __global__ void k_items<false>(const SysDev*, int, int64_t, const int32_t*, const int32_t*,

@@ -101,7 +101,9 @@ def main(argv=None):
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     sizes, grids, d = workload(a.workload, a.seed)
-    eng = Engine(sizes, grids)
+    from barneshut.batched import BATCHED_TUNING
+    # log and compare with the launch shape each workload runs with
+    eng = Engine(sizes, grids, tuning=BATCHED_TUNING if len(sizes) > 1 else None)
     eng.load(d)
     t0 = time.perf_counter()
     X, c = costmodel.collect(eng, a.steps, LAW, "fp32", dt=DT,

@@ -216,6 +216,29 @@ def test_fp32_mass_scale_exact_and_zero_mass_cells():
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_coincident_particles_softened(prec):
+    """Distinct particles at identical positions: with softening their mutual
+    terms are zero and finite (self is skipped by index, not by distance)."""
+    d = inputs.gaussian_mixture(20000, 6, (0.01, 0.1), seed=35)
+    rng = np.random.default_rng(36)
+    src = rng.choice(20000, 300, replace=False)
+    dst = rng.choice(np.setdiff1d(np.arange(20000), src), 300, replace=False)
+    d["px"][dst] = d["px"][src]
+    d["py"][dst] = d["py"][src]
+    G = 64
+    eng = _forces_setup(d, G)
+    eng.evaluate(1.0, 1e-7, 0.0, prec)
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, 1.0, 1e-7, 0.0)
+    g = eng.fetch()
+    assert np.isfinite(g["ax"]).all() and np.isfinite(g["ay"]).all()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    print(f"[coincident {prec}] normwise={norm:.3e} max={per.max():.3e}", flush=True)
+    assert norm <= (1e-12 if prec == "fp64" else 1e-5)
+    eng.close()
+
+
 @pytest.mark.parametrize("G", [1, 2])
 def test_small_grid_is_direct_sum(G):
     """G <= 2: every cell is a neighbour, so the method is the direct sum."""

@@ -208,7 +208,10 @@ def batched_line(steps, model_path, seed=0):
             model = costmodel.CostModel.load(model_path, device=torch.device("cuda"))
         b = BatchedSystems(sizes, grids, data, law, "fp32", DT, graph=True, model=model)
         sec = b.run(steps, warmup=1)
-        out[name] = {"ms_per_step": sec * 1e3, "value": float(sizes.sum()) / sec / 1e6}
+        inter = b.eng.interactions()[-1].astype(np.float64)    # placement of the last step
+        out[name] = {"ms_per_step": sec * 1e3, "value": float(sizes.sum()) / sec / 1e6,
+                     "interactions_last_step": float(inter.sum()),
+                     "tflops_whole_step": FLOP_PER_INTERACTION * float(inter.sum()) / sec / 1e12}
         b.close()
     return out
 

@@ -83,6 +83,11 @@ __global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ px,
         mny = fmin(mny, y);
         mxy = fmax(mxy, y);
     }
+    // Reduce over the block when all its rows belong to one system (one
+    // flush per block), else per wave, else per lane.
+    __shared__ double red[4][4];
+    __shared__ int wsys[4];
+    const int wid = threadIdx.x >> 6;
     int c0 = __shfl(cur, 0, BH_WAVE);
     bool uni = __all(cur == c0);
     if (uni && c0 >= 0) {
@@ -90,6 +95,29 @@ __global__ __launch_bounds__(256) void k_bbox(const double* __restrict__ px,
         mny = wave_min(mny);
         mxx = wave_max(mxx);
         mxy = wave_max(mxy);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        wsys[wid] = (uni && c0 >= 0) ? c0 : -1;
+        red[wid][0] = mnx;
+        red[wid][1] = mny;
+        red[wid][2] = mxx;
+        red[wid][3] = mxy;
+    }
+    __syncthreads();
+    const int nwv = (int)(blockDim.x >> 6);
+    bool block_uni = true;
+    for (int v = 0; v < nwv; ++v) block_uni = block_uni && wsys[v] >= 0 && wsys[v] == wsys[0];
+    if (block_uni) {
+        if (threadIdx.x == 0) {
+            for (int v = 1; v < nwv; ++v) {
+                mnx = fmin(mnx, red[v][0]);
+                mny = fmin(mny, red[v][1]);
+                mxx = fmax(mxx, red[v][2]);
+                mxy = fmax(mxy, red[v][3]);
+            }
+            bbox_flush(bb, wsys[0], mnx, mny, mxx, mxy);
+        }
+    } else if (uni && c0 >= 0) {
         if ((threadIdx.x & 63) == 0) bbox_flush(bb, c0, mnx, mny, mxx, mxy);
     } else if (cur >= 0) {
         bbox_flush(bb, cur, mnx, mny, mxx, mxy);

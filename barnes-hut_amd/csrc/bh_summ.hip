@@ -30,11 +30,21 @@ __global__ __launch_bounds__(256) void k_com(const SysDev* __restrict__ sys, int
                                              Com64* __restrict__ l64, Com32* __restrict__ l32)
 {
     const int lane = threadIdx.x & 63;
-    int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int nne = stats[ST_NCELLS_NE];
-    for (; w < nne; w += nw) {
+    // each wave takes a contiguous run of the non-empty list, so it crosses
+    // few system boundaries and looks its system up only when it does
+    const int64_t per = (nne + nw - 1) / nw;
+    const int64_t w0 = wv * per, w1 = min((int64_t)nne, w0 + per);
+    int s = -1;
+    int64_t s_end = -1;
+    for (int64_t w = w0; w < w1; ++w) {
         int c = ne_list[w];
+        if (c >= s_end) {
+            s = sys_of_cell(sys, S, c);
+            s_end = (s + 1 < S) ? sys[s + 1].c_off : INT64_MAX;
+        }
         int a = offsets[c], b = offsets[c + 1];
         double sx = 0.0, sy = 0.0, sm = 0.0;
         for (int i = a + lane; i < b; i += BH_WAVE) {
@@ -56,7 +66,7 @@ __global__ __launch_bounds__(256) void k_com(const SysDev* __restrict__ sys, int
             com_dense[3 * (int64_t)c + 0] = X;
             com_dense[3 * (int64_t)c + 1] = Y;
             com_dense[3 * (int64_t)c + 2] = M;
-            const SysDev& d = sys[sys_of_cell(sys, S, c)];
+            const SysDev& d = sys[s];
             int local = (int)(c - d.c_off);
             int kx = local / d.G, ky = local - kx * d.G;
             Com64 e64;

@@ -60,6 +60,7 @@ struct bh_ctx_s {
     int32_t *key = nullptr, *key_sorted = nullptr, *counts = nullptr, *cursor = nullptr;
     int32_t *offsets = nullptr, *neprefix = nullptr, *ne_list = nullptr;
     int32_t *idx_a = nullptr, *idx_b = nullptr, *perm = nullptr;
+    uint64_t *pxkey = nullptr, *pykey = nullptr;   // sort keys of the scattered rows
     int32_t *bsum = nullptr, *bne = nullptr;
     int32_t* stats = nullptr;
     double* com_dense = nullptr;
@@ -213,6 +214,8 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     A(ne_list, C);
     A(idx_a, N);
     A(idx_b, N);
+    A(pxkey, N);
+    A(pykey, N);
     A(perm, N);
     A(bsum, nb);
     A(bne, nb);
@@ -402,9 +405,11 @@ int bh_sort_particles(void* ctx, const bh_particles* src, const bh_particles* ds
                        c->C, 0);
     hipLaunchKernelGGL(k_scatter, dim3(cdiv(N, PLACE_CHUNK)), dim3(256), 0, st,
                        (const int32_t*)c->key, N, (const int32_t*)c->offsets, c->cursor, c->idx_a,
-                       c->key_sorted);
+                       c->key_sorted, (const double*)src->px, (const double*)src->py, c->pxkey,
+                       c->pykey);
     hipLaunchKernelGGL(k_sort_units, dim3(cdiv(N, SORT_W)), dim3(256), 0, st, F, N,
                        (const int32_t*)c->idx_a, (const int32_t*)c->key_sorted,
+                       (const uint64_t*)c->pxkey, (const uint64_t*)c->pykey,
                        (const int32_t*)c->offsets, c->idx_b);
     // merge passes for cells longer than SORT_W: b -> a -> b -> ...
     int maxpass = 0;

@@ -5,7 +5,7 @@
 #define PLACE_CHUNK 2048   // particles per block in bbox / keys / scatter
 #define HIST_BINS 8192     // LDS histogram window (keys) per block
 #define SCAN_TILE 2048     // elements per block in the scans
-#define SORT_W 2048        // within-cell rank window (rows)
+#define SORT_W 1024        // within-cell rank window (rows)
 
 struct SortFields {
     const double *px, *py, *m, *vx, *vy, *ax, *ay;
@@ -56,9 +56,11 @@ template <bool CNT>
 __global__ void k_scan_final(const int32_t* in, int64_t n, const int32_t* bsum, const int32_t* bne,
                              int32_t* out_v, int32_t* out_f);
 __global__ void k_scatter(const int32_t* key, int64_t N, const int32_t* offsets, int32_t* cursor,
-                          int32_t* idx_out, int32_t* key_sorted);
+                          int32_t* idx_out, int32_t* key_sorted, const double* px,
+                          const double* py, uint64_t* pxkey_out, uint64_t* pykey_out);
 __global__ void k_sort_units(SortFields F, int64_t N, const int32_t* idx_in,
-                             const int32_t* key_sorted, const int32_t* offsets, int32_t* idx_out);
+                             const int32_t* key_sorted, const uint64_t* pxkey,
+                             const uint64_t* pykey, const int32_t* offsets, int32_t* idx_out);
 __global__ void k_sort_merge(SortFields F, int64_t N, const int32_t* idx_in,
                              const int32_t* key_sorted, const int32_t* offsets, int32_t* idx_out,
                              int L, const int32_t* stats);

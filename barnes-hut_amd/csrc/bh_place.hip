@@ -383,15 +383,29 @@ template __global__ void k_scan_final<true>(const int32_t*, int64_t, const int32
 template __global__ void k_scan_final<false>(const int32_t*, int64_t, const int32_t*,
                                              const int32_t*, int32_t*, int32_t*);
 
+// 64-bit key ordered as numpy orders doubles: -0.0 equals +0.0, every NaN is
+// equal to every other NaN and sorts after all numbers.
+__device__ __forceinline__ uint64_t sort_key(double d)
+{
+    if (isnan(d)) return ~0ull;
+    return bh_ord_key(d == 0.0 ? 0.0 : d);
+}
+
 // ---------------------------------------------------------------- scatter
 // Counting-sort scatter into cell order.  A block reserves one contiguous
 // range per distinct key with a single global atomic (LDS-privatised
 // cursor); the order inside a cell is fixed afterwards by k_sort_units.
+// Each row's px and py sort keys travel with it, so that pass reads them
+// contiguously (rows clipped to the box edge tie in px by the thousand).
 __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ key, int64_t N,
                                                  const int32_t* __restrict__ offsets,
                                                  int32_t* __restrict__ cursor,
                                                  int32_t* __restrict__ idx_out,
-                                                 int32_t* __restrict__ key_sorted)
+                                                 int32_t* __restrict__ key_sorted,
+                                                 const double* __restrict__ px,
+                                                 const double* __restrict__ py,
+                                                 uint64_t* __restrict__ pxkey_out,
+                                                 uint64_t* __restrict__ pykey_out)
 {
     __shared__ int32_t hist[HIST_BINS];
     __shared__ int red[8];
@@ -428,8 +442,11 @@ __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ key
         for (int r = 0; r < R; ++r) {
             if (kk[r] >= 0) {
                 int pos = hist[kk[r] - kmin] + lr[r];
-                idx_out[pos] = (int32_t)(base + r * 256 + threadIdx.x);
+                const int64_t i = base + r * 256 + threadIdx.x;
+                idx_out[pos] = (int32_t)i;
                 key_sorted[pos] = kk[r];
+                pxkey_out[pos] = sort_key(px[i]);
+                pykey_out[pos] = sort_key(py[i]);
             }
         }
     } else {
@@ -437,8 +454,11 @@ __global__ __launch_bounds__(256) void k_scatter(const int32_t* __restrict__ key
         for (int r = 0; r < R; ++r) {
             if (kk[r] >= 0) {
                 int pos = offsets[kk[r]] + atomicAdd(&cursor[kk[r]], 1);
-                idx_out[pos] = (int32_t)(base + r * 256 + threadIdx.x);
+                const int64_t i = base + r * 256 + threadIdx.x;
+                idx_out[pos] = (int32_t)i;
                 key_sorted[pos] = kk[r];
+                pxkey_out[pos] = sort_key(px[i]);
+                pykey_out[pos] = sort_key(py[i]);
             }
         }
     }
@@ -476,34 +496,29 @@ __device__ __forceinline__ bool row_less(const SortFields& F, int a, int b)
     return tie_less(F, a, b);
 }
 
-// 64-bit key ordered as numpy orders doubles: -0.0 equals +0.0, every NaN is
-// equal to every other NaN and sorts after all numbers.
-__device__ __forceinline__ uint64_t sort_key(double d)
-{
-    if (isnan(d)) return ~0ull;
-    return bh_ord_key(d == 0.0 ? 0.0 : d);
-}
-
 // A "unit" is a whole cell when the cell holds <= SORT_W particles, else one
 // SORT_W-long chunk of it (chunks counted from the cell start).  Every unit
 // starts in exactly one aligned SORT_W window; the block owning that window
-// ranks the unit's rows in LDS (all of them lie in [w0, w0 + 2*SORT_W)).
+// ranks the unit's rows in LDS (all of them lie in [w0, w0 + 2*SORT_W)) on
+// their px keys, then py keys; the remaining fields are read only for rows
+// equal in both.
 __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
                                                     const int32_t* __restrict__ idx_in,
                                                     const int32_t* __restrict__ key_sorted,
+                                                    const uint64_t* __restrict__ pxkey,
+                                                    const uint64_t* __restrict__ pykey,
                                                     const int32_t* __restrict__ offsets,
                                                     int32_t* __restrict__ idx_out)
 {
-    __shared__ uint64_t lkey[2 * SORT_W];
-    __shared__ int32_t lidx[2 * SORT_W];
+    __shared__ uint64_t lkx[2 * SORT_W];
+    __shared__ uint64_t lky[2 * SORT_W];
     const int64_t w0 = (int64_t)blockIdx.x * SORT_W;
     const int64_t wend = min((int64_t)N, w0 + 2 * SORT_W);
     for (int t = threadIdx.x; t < 2 * SORT_W; t += 256) {
         int64_t q = w0 + t;
         if (q < wend) {
-            int e = idx_in[q];
-            lidx[t] = e;
-            lkey[t] = sort_key(F.px[e]);
+            lkx[t] = pxkey[q];
+            lky[t] = pykey[q];
         }
     }
     __syncthreads();
@@ -522,13 +537,18 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
             ub = min(b, ua + SORT_W);
         }
         if (ua < w0 || ua >= w0 + SORT_W) continue;   // unit owned by another block
-        int e = lidx[t];
-        const uint64_t x = lkey[t];
+        const int e = idx_in[q];
+        const uint64_t x = lkx[t], xy = lky[t];
         int rank = 0;
         for (int j = (int)(ua - w0); j < (int)(ub - w0); ++j) {
-            const uint64_t y = lkey[j];
-            if (y < x) ++rank;
-            else if (y == x && j != t && tie_less(F, lidx[j], e)) ++rank;
+            const uint64_t y = lkx[j];
+            if (y < x) {
+                ++rank;
+            } else if (y == x) {
+                const uint64_t yy = lky[j];
+                if (yy < xy) ++rank;
+                else if (yy == xy && j != t && tie_less(F, idx_in[w0 + j], e)) ++rank;
+            }
         }
         idx_out[ua + rank] = e;
     }

@@ -161,6 +161,25 @@ def test_forces_fp64(law):
     eng.close()
 
 
+@pytest.mark.parametrize("law", LAWS, ids=[l[0] for l in LAWS])
+def test_forces_fp32_all_laws(law):
+    """FP32 path under each law switch (the cutoff law keeps the unscaled far
+    term; the others use the mass-scaled form) against the FP64 oracle."""
+    _, Gc, eps2, r2 = law
+    d = inputs.gaussian_mixture(40000, 12, (0.005, 0.05), seed=23)
+    G = 64
+    eng = _forces_setup(d, G)
+    eng.evaluate(Gc, eps2, r2, "fp32")
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, Gc, eps2, r2)
+    g = eng.fetch()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    p99 = float(np.quantile(per, 0.99))
+    print(f"[fp32 {law[0]}] normwise={norm:.3e} p99={p99:.3e} max={per.max():.3e}", flush=True)
+    assert norm <= 1e-5 and p99 <= 1e-5
+    eng.close()
+
+
 @pytest.mark.parametrize("prec,tol_norm,tol_p99", [("fp32", 1e-5, 1e-5),
                                                    ("fp32_mfma", 1e-5, 1e-5),
                                                    ("fp32_kahan_term", 1e-5, 1e-5),

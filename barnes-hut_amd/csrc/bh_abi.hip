@@ -534,7 +534,8 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
     if (fa.split > 1) {
         const int pslots = 64 * c->tune.p;
-        hipLaunchKernelGGL(k_force_combine, dim3(cdiv((int64_t)fa.n_tail * pslots, 256)), dim3(256),
+        hipLaunchKernelGGL(k_force_combine, dim3(grid_for((int64_t)fa.n_tail * pslots, 256,
+                                                          (int64_t)c->num_cu * 32)), dim3(256),
                            0, st, (const Item*)c->items, order, (const int32_t*)c->stats, fa.split,
                            fa.n_tail, pslots, (const double*)c->part, fa.part_n, p->ax, p->ay);
     }

@@ -653,23 +653,25 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
     const int nitems = stats[ST_NITEMS];
     const int nt = min(n_tail, nitems);
     const int nu = nitems - nt;
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = (int)(g / pslots), slot = (int)(g - (int64_t)j * pslots);
-    if (j >= nt) return;
-    const int pos = nu + j;
-    const int itx = order ? order[pos] : pos;
-    if ((unsigned)itx >= (unsigned)nitems) return;
-    const Item it = items[itx];
-    if (slot >= it.np) return;
-    const int64_t gi = (int64_t)it.p0 + slot;
-    double sx = 0.0, sy = 0.0;
-    for (int s = 0; s < split; ++s) {
-        const int64_t o = ((int64_t)s * part_n + gi) * 2;
-        sx += part[o];
-        sy += part[o + 1];
+    const int64_t total = (int64_t)nt * pslots;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const int j = (int)(g / pslots), slot = (int)(g - (int64_t)j * pslots);
+        const int pos = nu + j;
+        const int itx = order ? order[pos] : pos;
+        if ((unsigned)itx >= (unsigned)nitems) continue;
+        const Item it = items[itx];
+        if (slot >= it.np) continue;
+        const int64_t gi = (int64_t)it.p0 + slot;
+        double sx = 0.0, sy = 0.0;
+        for (int s = 0; s < split; ++s) {
+            const int64_t o = ((int64_t)s * part_n + gi) * 2;
+            sx += part[o];
+            sy += part[o + 1];
+        }
+        ax[gi] += sx;
+        ay[gi] += sy;
     }
-    ax[gi] += sx;
-    ay[gi] += sy;
 }
 
 // ---------------------------------------------------------------- FP32 + matrix cores

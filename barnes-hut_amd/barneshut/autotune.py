@@ -16,7 +16,7 @@ import json
 import os
 
 DEFAULT_CANDIDATES = dict(p=(1, 2, 4), yspan=(128, 512), waves_per_cu=(16, 32),
-                          tail_split=(1, 8))
+                          tail_split=(1, 8, 16))
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "bh_mi355x_autotune.json")
 
 

@@ -47,7 +47,7 @@ class HipBarnesHut(BaseBarnesHut):
         self.tuning = dict(p=int(Config.get("hip", "p", fallback="2")),
                            yspan=int(Config.get("hip", "yspan", fallback="512")),
                            waves_per_cu=int(Config.get("hip", "waves_per_cu", fallback="32")),
-                           tail_split=int(Config.get("hip", "tail_split", fallback="8")),
+                           tail_split=int(Config.get("hip", "tail_split", fallback="16")),
                            tail_items=int(Config.get("hip", "tail_items", fallback="1048576")))
         self.device = int(Config.get("hip", "device", fallback="0"))
         self.engine = None

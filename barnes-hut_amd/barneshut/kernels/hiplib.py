@@ -185,7 +185,7 @@ class Context:
         self.n = n.copy()
         self.grid_dim = g.copy()
 
-    def set_tuning(self, p=2, yspan=512, waves_per_cu=32, tail_split=8, tail_items=1 << 20):
+    def set_tuning(self, p=2, yspan=512, waves_per_cu=32, tail_split=16, tail_items=1 << 20):
         t = BhTuning(int(p), int(yspan), int(waves_per_cu), int(tail_split), int(tail_items))
         check(self.L.bh_set_tuning(self.h, ctypes.byref(t)))
 

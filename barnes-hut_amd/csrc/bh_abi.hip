@@ -76,7 +76,7 @@ struct bh_ctx_s {
     int32_t* cost_scratch = nullptr;
     float* feat = nullptr;
     float* pred = nullptr;
-    bh_tuning tune = {2, 512, 32, 8, 1 << 20};
+    bh_tuning tune = {2, 512, 32, 16, 1 << 20};
     double* part = nullptr;
     size_t part_cap = 0;
     double* energy_buf = nullptr;   // 2 x N, allocated by the first bh_energy
@@ -269,8 +269,8 @@ int bh_set_tuning(void* ctx, const bh_tuning* t)
     if (t->yspan < 1 || t->yspan > 1 << 20) return fail(BH_EINVAL, "tuning.yspan must be >= 1");
     if (t->waves_per_cu < 1 || t->waves_per_cu > 64)
         return fail(BH_EINVAL, "tuning.waves_per_cu must be in [1, 64]");
-    if (t->tail_split < 1 || t->tail_split > 8)
-        return fail(BH_EINVAL, "tuning.tail_split must be in [1, 8]");
+    if (t->tail_split < 1 || t->tail_split > 16)
+        return fail(BH_EINVAL, "tuning.tail_split must be in [1, 16]");
     if (t->tail_items < 0 || t->tail_items > (1 << 20))
         return fail(BH_EINVAL, "tuning.tail_items must be in [0, 2^20]");
     c->tune = *t;

@@ -256,7 +256,7 @@ def main():
     ap.add_argument("--p", type=int, default=2, help="FP32 path: particles per lane (1, 2, 4)")
     ap.add_argument("--yspan", type=int, default=512, help="max rows per work item")
     ap.add_argument("--waves-per-cu", type=int, default=32)
-    ap.add_argument("--tail-split", type=int, default=8,
+    ap.add_argument("--tail-split", type=int, default=16,
                     help="FP32 path: far-field slices per split item (1 = off)")
     ap.add_argument("--tail-items", type=int, default=1 << 20,
                     help="FP32 path: split the last this many queue items (>= items: all)")

@@ -67,11 +67,11 @@ typedef struct bh_law {
 /* Launch shape of the force kernel. p: particles per lane (1, 2 or 4);
  * yspan: max rows per work item (>= 1); waves_per_cu: persistent waves per
  * compute unit (>= 1); tail_split: the FP32 kernels run each of the last
- * tail_items queue entries as tail_split far-field slices (1 = off, <= 8;
+ * tail_items queue entries as tail_split far-field slices (1 = off, <= 16;
  * tail_items >= the item count splits every item), so the queue drains in
  * finer pieces; the slices' sums are added in a fixed order by a second
  * kernel (scratch: particles x tail_split x 16 bytes).
- * Default: p 2, yspan 512, waves_per_cu 32, tail_split 8, tail_items 2^20. */
+ * Default: p 2, yspan 512, waves_per_cu 32, tail_split 16, tail_items 2^20. */
 typedef struct bh_tuning {
     int32_t p;
     int32_t yspan;

@@ -45,6 +45,7 @@ struct GraphKey {
     bh_tuning tune;
     const float* params;
     int32_t* order;
+    const double* part;   // slice-plane buffer the graph was captured with
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -252,6 +253,12 @@ static int ensure_part(Ctx* c)
                       : 0;
     if (need > c->part_cap) {
         HIP_TRY(hipDeviceSynchronize());
+        // captured steps hold the old buffer's address: drop them
+        for (auto& g : c->gexec)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
         if (c->part) (void)hipFree(c->part);
         c->part = nullptr;
         c->part_cap = 0;
@@ -605,6 +612,7 @@ static int step_graph_impl(void* ctx, const bh_particles* a, const bh_particles*
     k.tune = c->tune;
     k.params = params;
     k.order = order;
+    k.part = c->part;
     int slot = -1;
     for (int i = 0; i < 2; ++i)
         if (c->gexec[i] && c->gkey[i] == k) slot = i;

@@ -353,6 +353,35 @@ def test_step_and_graph_match_phases():
         e.close()
 
 
+def test_graph_recaptured_when_slice_buffer_grows():
+    """FP32 graph steps under launch shapes whose slice-plane buffer grows and
+    then returns to an earlier shape: every replay equals the eager step with
+    the same shape (a captured step must never keep a freed buffer)."""
+    d = inputs.gaussian_mixture(30000, 8, (0.01, 0.05), seed=14)
+    G = 64
+    kw = dict(G=1.0, eps2=1e-6, r2_cut=0.0, precision="fp32", dt=1e-4)
+    shapes = [dict(p=2, yspan=64, waves_per_cu=8, tail_split=2, tail_items=1 << 20),
+              dict(p=2, yspan=64, waves_per_cu=8, tail_split=16, tail_items=1 << 20),
+              dict(p=2, yspan=64, waves_per_cu=8, tail_split=2, tail_items=1 << 20)]
+    e1 = _engine(len(d["px"]), G)
+    e2 = _engine(len(d["px"]), G)
+    for e in (e1, e2):
+        e.load(d)
+    for tun in shapes:
+        print(f"[graph shapes] {tun}", flush=True)
+        for e in (e1, e2):
+            e.ctx.set_tuning(**tun)
+        for _ in range(2):
+            e1.step(**kw)
+            e2.step(graph=True, **kw)
+            e2.check()
+            a, b = e1.fetch(), e2.fetch()
+            for k in ("px", "py", "vx", "vy", "pid"):
+                np.testing.assert_array_equal(a[k], b[k])
+    for e in (e1, e2):
+        e.close()
+
+
 def test_multi_step_against_oracle():
     """Three full steps (place, summarize, evaluate, tick) against the oracle
     step: placement exact at every step, positions within FP64 tolerance."""

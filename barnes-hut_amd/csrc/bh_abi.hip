@@ -46,6 +46,7 @@ struct GraphKey {
     const float* params;
     int32_t* order;
     const double* part;   // slice-plane buffer the graph was captured with
+    int cost_log_on;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -613,6 +614,7 @@ static int step_graph_impl(void* ctx, const bh_particles* a, const bh_particles*
     k.params = params;
     k.order = order;
     k.part = c->part;
+    k.cost_log_on = c->cost_log_on;
     int slot = -1;
     for (int i = 0; i < 2; ++i)
         if (c->gexec[i] && c->gkey[i] == k) slot = i;

@@ -281,8 +281,16 @@ int bh_set_tuning(void* ctx, const bh_tuning* t)
         return fail(BH_EINVAL, "tuning.tail_split must be in [1, 16]");
     if (t->tail_items < 0 || t->tail_items > (1 << 20))
         return fail(BH_EINVAL, "tuning.tail_items must be in [0, 2^20]");
+    const bh_tuning old = c->tune;
     c->tune = *t;
-    return ensure_part(c);
+    int r = ensure_part(c);
+    if (r != BH_OK) {   // keep a consistent state: the previous shape and its buffer
+        std::string msg = g_err;
+        c->tune = old;
+        (void)ensure_part(c);
+        return fail(r, msg);
+    }
+    return BH_OK;
 }
 
 int bh_get_tuning(void* ctx, bh_tuning* t)
@@ -534,6 +542,8 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     const bool fp32v = precision >= 1 && precision <= 3;
     fa.split = (fp32v && c->tune.tail_split > 1 && c->tune.tail_items > 0) ? c->tune.tail_split : 1;
     fa.n_tail = fa.split > 1 ? c->tune.tail_items : 0;
+    if (fa.split > 1 && c->part_cap < (size_t)c->N * fa.split * 2 * sizeof(double))
+        return fail(BH_EINVAL, "slice buffer missing for tail_split (bh_set_tuning failed?)");
     fa.part = c->part;
     fa.part_n = c->N;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);

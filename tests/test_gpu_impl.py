@@ -151,3 +151,22 @@ def test_invalid_arguments_raise():
     with pytest.raises(BhError):
         e.ctx.sort(e.parts, e.parts)
     e.close()
+
+
+@pytest.mark.parametrize("n", [1, 64])
+def test_degenerate_box_raises(n):
+    """All particles at one point (or a single particle): the bounding box has
+    zero extent, the cell size is 0; keys stay inside the grid and the
+    context reports the condition instead of producing garbage cells."""
+    from barneshut.engine import Engine
+    from barneshut.kernels.hiplib import BhError
+    d = inputs.uniform(n, seed=9)
+    d["px"][:] = 0.3
+    d["py"][:] = 0.7
+    e = Engine([n], [4])
+    e.load(d)
+    e.create_tree()
+    with pytest.raises(BhError, match="zero-extent"):
+        e.check()
+    assert (e.keys_in() >= 0).all() and (e.keys_in() < 16).all()
+    e.close()

@@ -24,12 +24,25 @@ BATCHED_TUNING = dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=
 class BatchedSystems:
 
     def __init__(self, sizes, grid_dims, data, law=(1.0, 0.0, 0.0), precision="fp32", dt=0.0,
-                 tuning=None, device=0, model=None, graph=True):
+                 tuning=None, device=0, model=None, graph=True, autotune_candidates=None):
+        """tuning: None (BATCHED_TUNING), a dict, or "auto": the autotuner picks
+        the launch shape on the loaded state (autotune_candidates narrows it)."""
         self.sizes = np.asarray(sizes, np.int64)
         self.grid_dims = np.asarray(grid_dims, np.int32)
+        auto = isinstance(tuning, str)
+        if auto and tuning != "auto":
+            raise ValueError(f"tuning must be None, a dict or 'auto', not {tuning!r}")
         self.eng = Engine(self.sizes, self.grid_dims, device=device,
-                          tuning=BATCHED_TUNING if tuning is None else tuning)
+                          tuning=BATCHED_TUNING if (tuning is None or auto) else tuning)
         self.eng.load(data)
+        self.tune_result = None
+        if auto:
+            from . import autotune
+            self.eng.create_tree()
+            self.eng.summarize()
+            self.tune_result = autotune.tune(self.eng, law, precision,
+                                             candidates=autotune_candidates)
+            self.eng.load(data)
         self.law = law
         self.precision = precision
         self.dt = dt

@@ -190,9 +190,14 @@ def cpu_baseline(d, G, budget_s=12.0):
             "seconds_per_step": t_step}
 
 
+# launch shapes the autotuner tries for the batched line (untimed, cached)
+BATCHED_CANDIDATES = dict(p=(1, 2), yspan=(512,), waves_per_cu=(16, 32), tail_split=(1, 16))
+
+
 def batched_line(steps, model_path, seed=0):
     """512 independent uniform systems (N log-uniform [1024, 262144], grid
-    side ceil(sqrt(N/16))), FP32, one graph launch per step."""
+    side ceil(sqrt(N/16))), FP32, one graph launch per step, launch shape
+    picked by the autotuner on the loaded state (not timed)."""
     from barneshut import costmodel, inputs
     from barneshut.batched import BatchedSystems
     sizes, grids = inputs.batched_sizes(512, 1024, 262144, seed=seed)
@@ -206,7 +211,9 @@ def batched_line(steps, model_path, seed=0):
                 continue
             import torch
             model = costmodel.CostModel.load(model_path, device=torch.device("cuda"))
-        b = BatchedSystems(sizes, grids, data, law, "fp32", DT, graph=True, model=model)
+        b = BatchedSystems(sizes, grids, data, law, "fp32", DT, graph=True, model=model,
+                           tuning="auto", autotune_candidates=BATCHED_CANDIDATES)
+        out["tuning"] = b.tune_result["tuning"]
         sec = b.run(steps, warmup=1)
         inter = b.eng.interactions()[-1].astype(np.float64)    # placement of the last step
         out[name] = {"ms_per_step": sec * 1e3, "value": float(sizes.sum()) / sec / 1e6,

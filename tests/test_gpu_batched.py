@@ -72,3 +72,55 @@ def test_each_system_matches_oracle_fp32():
         assert np.sqrt((num ** 2).sum() / (den ** 2).sum()) < 1e-5, s
     assert off[-1] == e.N
     b.close()
+
+
+def test_autotuner_picks_admissible_shape_and_caches(tmp_path):
+    """The autotuner times every candidate on the loaded state, admits only
+    FP32 shapes within max_rel_err of an FP64 pass, sets the fastest, and
+    answers the same request from its cache."""
+    from barneshut import autotune
+    from barneshut.engine import Engine
+    sizes, grids = inputs.batched_sizes(24, 1024, 65536, seed=4)
+    data = inputs.batched_uniform(sizes, seed=4)
+    e = Engine(sizes, grids)
+    e.load(data)
+    e.create_tree()
+    e.summarize()
+    cand = dict(p=(1, 2), yspan=(64, 512), waves_per_cu=(16,), tail_split=(1, 8))
+    cache = str(tmp_path / "tune.json")
+    law = (1e-6, 1e-12, 0.0)
+    r = autotune.tune(e, law, "fp32", candidates=cand, max_rel_err=1e-5, cache=cache)
+    assert len(r["table"]) == 8
+    assert all(row["rel_err"] <= 1e-5 for row in r["table"])
+    best = min(r["table"], key=lambda row: row["ms"])
+    assert r["tuning"] == best["tuning"]
+    assert e.ctx.get_tuning() == r["tuning"]
+    print(f"[autotune] best {r['tuning']} {r['ms']:.3f} ms", flush=True)
+    hit = autotune.tune(e, law, "fp32", candidates=cand, cache=cache)
+    assert hit["tuning"] == r["tuning"] and "table" not in hit
+    with pytest.raises(RuntimeError):     # an impossible error bound admits nothing
+        autotune.tune(e, law, "fp32", candidates=cand, max_rel_err=1e-15, use_cache=False,
+                      cache=None)
+    e.close()
+
+
+def test_batched_systems_autotuned_step_matches_eager():
+    """BatchedSystems(tuning="auto") runs with the tuner's shape, and its graph
+    steps equal eager steps under that shape."""
+    sizes, grids = inputs.batched_sizes(16, 1024, 32768, seed=5)
+    data = inputs.batched_uniform(sizes, seed=5)
+    law = (1e-6, 1e-8, 0.0)
+    cand = dict(p=(1, 2), yspan=(512,), waves_per_cu=(16,), tail_split=(1,))
+    b = BatchedSystems(sizes, grids, data, law, "fp32", 1e-4, tuning="auto",
+                       autotune_candidates=cand, graph=True)
+    tun = b.eng.ctx.get_tuning()
+    assert tun == b.tune_result["tuning"]
+    e = BatchedSystems(sizes, grids, data, law, "fp32", 1e-4, tuning=tun, graph=False)
+    for _ in range(2):
+        b.step()
+        e.step()
+    g1, g2 = b.eng.fetch(), e.eng.fetch()
+    for k in ("px", "py", "vx", "vy", "pid"):
+        np.testing.assert_array_equal(g1[k], g2[k])
+    b.close()
+    e.close()

@@ -258,6 +258,28 @@ def test_coincident_particles_softened(prec):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_small_cpu_case_method_error_against_direct_sum(prec):
+    """The reference's CPU-runnable case (16384 uniform particles, masses
+    U[1,2], 32x32 grid, softened law eps = 1e-3): the grid method's own error
+    against the FP64 direct sum, reported, is the oracle method's error (the
+    GPU path adds nothing measurable to the approximation)."""
+    d = inputs.uniform(16384, seed=1)
+    G = 32
+    eng = _forces_setup(d, G)
+    eng.evaluate(1.0, 1e-6, 0.0, prec)
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, 1.0, 1e-6, 0.0)
+    g = eng.fetch()
+    dx, dy = orc.direct(h["px"], h["py"], h["mass"], 1.0, 1e-6, 0.0)
+    e_gpu, _ = _rel(g["ax"], g["ay"], dx, dy)
+    e_orc, _ = _rel(rx, ry, dx, dy)
+    print(f"[small CPU case {prec}] method vs direct sum: GPU {e_gpu:.6e}, oracle {e_orc:.6e}",
+          flush=True)
+    assert abs(e_gpu - e_orc) <= 1e-5 * e_orc + (1e-12 if prec == "fp64" else 1e-6)
+    eng.close()
+
+
 @pytest.mark.parametrize("G", [1, 2])
 def test_small_grid_is_direct_sum(G):
     """G <= 2: every cell is a neighbour, so the method is the direct sum."""

@@ -15,8 +15,7 @@ __global__ void k_ne_list(const int32_t* __restrict__ counts, const int32_t* __r
         if (counts[c] > 0) ne_list[neprefix[c]] = (int32_t)c;
 }
 
-// One wave per non-empty cell: FP64 sums of m*x, m*y, m over the cell's
-// sorted particles, butterfly-reduced (fixed order, deterministic).
+// FP64 sums of m*x, m*y, m over each non-empty cell's sorted particles.
 // Writes the dense G*G x 3 array of the reference layout (empty cells were
 // zero-filled) and the two compact lists the force kernels stream.
 __global__ __launch_bounds__(256) void k_com(const SysDev* __restrict__ sys, int S,
@@ -29,34 +28,44 @@ __global__ __launch_bounds__(256) void k_com(const SysDev* __restrict__ sys, int
                                              double* __restrict__ com_dense,
                                              Com64* __restrict__ l64, Com32* __restrict__ l32)
 {
+    // A wave works on four cells at a time, one per 16-lane quarter (most
+    // cells hold a few dozen particles at most); the quarter's sums are
+    // butterfly-reduced inside the quarter (fixed order, deterministic).
+    constexpr int QW = 16;
     const int lane = threadIdx.x & 63;
+    const int q = lane >> 4, ql = lane & (QW - 1);
     const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
     const int nne = stats[ST_NCELLS_NE];
     // each wave takes a contiguous run of the non-empty list, so it crosses
     // few system boundaries and looks its system up only when it does
-    const int64_t per = (nne + nw - 1) / nw;
+    const int64_t per = (((nne + nw - 1) / nw) + 3) & ~(int64_t)3;
     const int64_t w0 = wv * per, w1 = min((int64_t)nne, w0 + per);
     int s = -1;
     int64_t s_end = -1;
-    for (int64_t w = w0; w < w1; ++w) {
-        int c = ne_list[w];
-        if (c >= s_end) {
+    for (int64_t wb = w0; wb < w1; wb += 4) {
+        const int64_t w = wb + q;
+        const bool live = w < w1;
+        const int c = live ? ne_list[w] : 0;
+        if (live && c >= s_end) {
             s = sys_of_cell(sys, S, c);
             s_end = (s + 1 < S) ? sys[s + 1].c_off : INT64_MAX;
         }
-        int a = offsets[c], b = offsets[c + 1];
+        const int a = live ? offsets[c] : 0, b = live ? offsets[c + 1] : 0;
         double sx = 0.0, sy = 0.0, sm = 0.0;
-        for (int i = a + lane; i < b; i += BH_WAVE) {
+        for (int i = a + ql; i < b; i += QW) {
             double mi = m[i];
             sx = fma(x[i], mi, sx);
             sy = fma(y[i], mi, sy);
             sm += mi;
         }
-        sx = wave_sum(sx);
-        sy = wave_sum(sy);
-        sm = wave_sum(sm);
-        if (lane == 0) {
+#pragma unroll
+        for (int o = QW / 2; o > 0; o >>= 1) {
+            sx += __shfl_xor(sx, o, BH_WAVE);
+            sy += __shfl_xor(sy, o, BH_WAVE);
+            sm += __shfl_xor(sm, o, BH_WAVE);
+        }
+        if (live && ql == 0) {
             double X = 0.0, Y = 0.0, M = 0.0;
             if (sm != 0.0) {
                 X = sx / sm;

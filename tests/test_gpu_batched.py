@@ -124,3 +124,50 @@ def test_batched_systems_autotuned_step_matches_eager():
         np.testing.assert_array_equal(g1[k], g2[k])
     b.close()
     e.close()
+
+
+def test_full_size_512_systems_sampled_against_oracle():
+    """The 512-system batch at full size (27.5M particles, one graph step of
+    the bench's line): for the smallest, a middle and the largest system,
+    placement keys and counts bit-exact against the oracle, rows sorted in
+    the reference's order, COMs and sampled FP32 accelerations within
+    tolerance."""
+    from barneshut.batched import BATCHED_TUNING
+    sizes, grids = inputs.batched_sizes(512, 1024, 262144, seed=0)
+    data = inputs.batched_uniform(sizes, seed=0)
+    law = (1e-6, 1e-12, 0.0)
+    b = BatchedSystems(sizes, grids, data, law, "fp32", 0.0, tuning=BATCHED_TUNING, graph=True)
+    e = b.eng
+    e.create_tree()
+    e.summarize()
+    e.evaluate(*law, "fp32")
+    e.check()
+    g = e.fetch()
+    keys = e.keys()
+    keys_in = e.keys_in()
+    coff = np.concatenate([[0], np.cumsum(grids.astype(np.int64) ** 2)])
+    slices = b.system_slices()
+    order = np.argsort(sizes, kind="stable")
+    rng = np.random.default_rng(8)
+    for s in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        sl, G = slices[s], int(grids[s])
+        n = sl.stop - sl.start
+        px, py = data["px"][sl], data["py"][sl]
+        bb = orc.bbox(px, py, G)
+        _, _, key = orc.place(px, py, bb[0], bb[1], bb[4], G)
+        np.testing.assert_array_equal(keys_in[sl] - coff[s], key)
+        ks = keys[sl] - coff[s]
+        assert (np.diff(ks) >= 0).all()
+        cnt, off = orc.counts_offsets(ks, G * G)
+        np.testing.assert_array_equal(cnt, np.bincount(key, minlength=G * G))
+        x, y, m = g["px"][sl], g["py"][sl], g["mass"][sl]
+        com = orc.summarize(off, x, y, m)
+        np.testing.assert_allclose(e.com().reshape(-1, 3)[coff[s]:coff[s + 1]], com,
+                                   rtol=1e-12, atol=1e-15)
+        idx = np.sort(rng.choice(n, min(n, 512), replace=False))
+        rx, ry = orc.forces(G, off, ks, x, y, m, com, *law, targets=idx)
+        num = np.hypot(g["ax"][sl][idx] - rx, g["ay"][sl][idx] - ry)
+        norm = float(np.sqrt((num ** 2).sum() / (np.hypot(rx, ry) ** 2).sum()))
+        print(f"[512 systems] system {s}: n={n} G={G} sampled normwise {norm:.3e}", flush=True)
+        assert norm <= 1e-5
+    b.close()

@@ -190,3 +190,41 @@ def test_large_full_size_one_step():
           f"p99 {np.quantile(num / den, 0.99):.3e}, max {(num / den).max():.3e}", flush=True)
     assert norm <= 1e-5
     eng.close()
+
+
+def test_learned_workload_full_size_sampled_against_oracle():
+    """The cost model's training workload at full size (N = 4,194,304, 256
+    Gaussian clusters, sigma log-uniform [0.001, 0.03], lognormal masses,
+    1024 x 1024 grid): one FP32 pass in the learned order; keys bit-exact,
+    rows sorted, COMs and sampled accelerations against the oracle."""
+    import torch
+    from barneshut import costmodel
+    n, G = 4194304, 1024
+    d = inputs.gaussian_mixture(n, 256, SIGMA, seed=91, mass="lognormal")
+    eng = _engine(n, G)
+    eng.load(d)
+    model = costmodel.CostModel.load(MODEL, device=torch.device("cuda"))
+    order = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
+    eng.create_tree()
+    eng.summarize()
+    eng.ctx.build_items()
+    eng.ctx.learned_order(model.params, order)
+    eng.ctx.force_pass(eng.parts, GC, 0.0, 0.0, "fp32", order)
+    eng.check()
+    bb = orc.bbox(d["px"], d["py"], G)
+    _, _, key = orc.place(d["px"], d["py"], bb[0], bb[1], bb[4], G)
+    np.testing.assert_array_equal(eng.keys_in(), key)
+    h = eng.fetch()
+    ks = eng.keys()
+    assert _sorted_under_total_order(h, ks)
+    off = eng.offsets().astype(np.int64)
+    ref = orc.summarize(off, h["px"], h["py"], h["mass"])
+    np.testing.assert_allclose(eng.com(), ref, rtol=1e-12, atol=1e-15)
+    idx = np.sort(np.random.default_rng(6).choice(n, 512, replace=False))
+    rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, 0.0, 0.0, targets=idx)
+    num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
+    norm = float(np.sqrt((num ** 2).sum() / (np.hypot(rx, ry) ** 2).sum()))
+    print(f"[learned workload] sampled normwise {norm:.3e}, p99 "
+          f"{np.quantile(num / np.hypot(rx, ry), 0.99):.3e}", flush=True)
+    assert norm <= 1e-5
+    eng.close()

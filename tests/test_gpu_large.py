@@ -192,15 +192,21 @@ def test_large_full_size_one_step():
     eng.close()
 
 
-def test_learned_workload_full_size_sampled_against_oracle():
-    """The cost model's training workload at full size (N = 4,194,304, 256
-    Gaussian clusters, sigma log-uniform [0.001, 0.03], lognormal masses,
-    1024 x 1024 grid): one FP32 pass in the learned order; keys bit-exact,
-    rows sorted, COMs and sampled accelerations against the oracle."""
+FULL_4M = [("bench_64_clusters", 64, (0.005, 0.05), "uniform", 2024),
+           ("learned_256_clusters_lognormal", 256, SIGMA, "lognormal", 91)]
+
+
+@pytest.mark.parametrize("name,nc,sigma,mass,seed", FULL_4M, ids=[w[0] for w in FULL_4M])
+def test_4m_workloads_full_size_sampled_against_oracle(name, nc, sigma, mass, seed):
+    """The 4,194,304-particle workloads at full size on the 1024 x 1024 grid
+    (the bench's: 64 clusters, sigma [0.005, 0.05], masses U[1,2]; the cost
+    model's: 256 clusters, sigma [0.001, 0.03], lognormal masses): one FP32
+    pass in the learned order; keys bit-exact, rows sorted, COMs and sampled
+    accelerations against the oracle."""
     import torch
     from barneshut import costmodel
     n, G = 4194304, 1024
-    d = inputs.gaussian_mixture(n, 256, SIGMA, seed=91, mass="lognormal")
+    d = inputs.gaussian_mixture(n, nc, sigma, seed=seed, mass=mass)
     eng = _engine(n, G)
     eng.load(d)
     model = costmodel.CostModel.load(MODEL, device=torch.device("cuda"))
@@ -224,7 +230,7 @@ def test_learned_workload_full_size_sampled_against_oracle():
     rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, 0.0, 0.0, targets=idx)
     num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
     norm = float(np.sqrt((num ** 2).sum() / (np.hypot(rx, ry) ** 2).sum()))
-    print(f"[learned workload] sampled normwise {norm:.3e}, p99 "
+    print(f"[{name}] sampled normwise {norm:.3e}, p99 "
           f"{np.quantile(num / np.hypot(rx, ry), 0.99):.3e}", flush=True)
     assert norm <= 1e-5
     eng.close()

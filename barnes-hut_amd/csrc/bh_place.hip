@@ -512,8 +512,20 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
 {
     __shared__ uint64_t lkx[2 * SORT_W];
     __shared__ uint64_t lky[2 * SORT_W];
+    __shared__ int64_t lend;
     const int64_t w0 = (int64_t)blockIdx.x * SORT_W;
-    const int64_t wend = min((int64_t)N, w0 + 2 * SORT_W);
+    if (threadIdx.x == 0) {
+        // rows past the window belong to this block only up to the end of the
+        // unit that holds the window's last row
+        const int64_t ql = min((int64_t)N, w0 + SORT_W) - 1;
+        const int k = key_sorted[ql];
+        const int64_t a = offsets[k], b = offsets[k + 1];
+        int64_t ub = b;
+        if (b - a > SORT_W) ub = min(b, a + ((ql - a) / SORT_W) * SORT_W + SORT_W);
+        lend = max(ql + 1, ub);
+    }
+    __syncthreads();
+    const int64_t wend = min(lend, w0 + 2 * SORT_W);
     for (int t = threadIdx.x; t < 2 * SORT_W; t += 256) {
         int64_t q = w0 + t;
         if (q < wend) {

@@ -25,6 +25,7 @@ PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32_KAHAN, "fp32_kahan": PREC_FP3
 MLP_NPARAM = 1377
 MLP_NFEAT = 8
 ST_ERR, ST_MAXCOUNT, ST_NITEMS, ST_QUEUE, ST_NCELLS_NE = 0, 1, 2, 3, 4
+ST_NRECOMP, ST_RCGROUPS = 6, 7
 
 # every entry point declared in include/bh_hip.h
 EXPORTED = (
@@ -33,7 +34,8 @@ EXPORTED = (
     "bh_sort_particles", "bh_summarize", "bh_build_items", "bh_evaluate_boxes",
     "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
-    "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy",
+    "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy", "bh_set_recompute",
+    "bh_recompute",
 )
 
 
@@ -107,6 +109,8 @@ def load(path: str = LIB_PATH):
         "bh_learned_order": (ctypes.c_int, [vp, vp, vp, vp]),
         "bh_query": (ctypes.c_int, [vp, i32, P(i64)]),
         "bh_energy": (ctypes.c_int, [vp, P(BhParticles), dbl, dbl, P(dbl), vp]),
+        "bh_set_recompute": (ctypes.c_int, [vp, dbl, i32]),
+        "bh_recompute": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -197,6 +201,19 @@ class Context:
 
     def set_cost_log(self, on: bool):
         check(self.L.bh_set_cost_log(self.h, 1 if on else 0))
+
+    def set_recompute(self, kappa=2.0, deferred=False):
+        """FP64 recompute after FP32 passes of the particles whose rounding-
+        error bound sqrt(sum |term|^2) exceeds kappa * |a| (kappa <= 0: off).
+        deferred: force_pass stops after the FP32 kernels and the caller runs
+        recompute() (the step calls still run it)."""
+        check(self.L.bh_set_recompute(self.h, float(kappa), 1 if deferred else 0))
+
+    def recompute(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, stream=None):
+        p = particles_struct(parts)
+        law = BhLaw(float(G), float(eps2), float(r2_cut))
+        check(self.L.bh_recompute(self.h, ctypes.byref(p), ctypes.byref(law),
+                                  stream_handle(stream)))
 
     def query(self, what: int) -> int:
         out = ctypes.c_int64()

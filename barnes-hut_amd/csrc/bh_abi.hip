@@ -47,6 +47,8 @@ struct GraphKey {
     int32_t* order;
     const double* part;   // slice-plane buffer the graph was captured with
     int cost_log_on;
+    double rc_kappa;
+    int rc_deferred;
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -82,6 +84,20 @@ struct bh_ctx_s {
     double* part = nullptr;
     size_t part_cap = 0;
     double* energy_buf = nullptr;   // 2 x N, allocated by the first bh_energy
+    // FP64 recompute: threshold, the FP32 pass's per-particle scratch, the
+    // flagged list and groups, per-block sums of M^2
+    double rc_kappa = 2.0;
+    int rc_deferred = 0;
+    int rc_ready = 0;        // the last force pass prepared a32 / s2n
+    double* rc_a32 = nullptr;
+    double* rc_s2n = nullptr;
+    int32_t* rc_list = nullptr;    // flagged particles, system s at [p_off_s, p_off_s + cnt_s)
+    int32_t* rc_cnt = nullptr;     // per system
+    int32_t* rc_gofs = nullptr;    // per system: first group (S + 1)
+    double* rc_part = nullptr;     // (group, slice) partial sums
+    double* rc_blk = nullptr;      // per block: sum of M^2
+    double* rc_bfar = nullptr;     // per block: its far-block part of the bound
+    int64_t nblocks = 0;
     hipStream_t cap_stream = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -175,8 +191,9 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     HIP_TRY(hipDeviceSynchronize());
     free_all(c);
     c->energy_buf = nullptr;
+    c->rc_ready = 0;
     c->sys_h.assign(nsys, SysDev{});
-    int64_t N = 0, C = 0, cols = 0, maxN = 0;
+    int64_t N = 0, C = 0, cols = 0, maxN = 0, NB = 0;
     for (int s = 0; s < nsys; ++s) {
         if (n[s] < 1) return fail(BH_EINVAL, "system " + std::to_string(s) + " has n < 1");
         if (grid_dim[s] < 1 || grid_dim[s] > 46340)
@@ -187,6 +204,9 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
         d.c_off = C;
         d.col_off = cols;
         d.G = grid_dim[s];
+        d.Gb = (int32_t)cdiv(grid_dim[s], RC_BLOCK);
+        d.b_off = NB;
+        NB += (int64_t)d.Gb * d.Gb;
         N += n[s];
         C += (int64_t)grid_dim[s] * grid_dim[s];
         cols += grid_dim[s];
@@ -199,6 +219,7 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     c->C = C;
     c->ncols = cols;
     c->maxN = maxN;
+    c->nblocks = NB;
     c->max_items = C + cols + 2 * cdiv(N, 64) + 64;
     int64_t nb = std::max(cdiv(C, SCAN_TILE), cdiv(cols, SCAN_TILE)) + 1;
     int r = BH_OK;
@@ -233,6 +254,14 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     A(cost_scratch, 16 + 3 * 4096 + 1);
     A(feat, BH_MLP_NFEAT * c->max_items);
     A(pred, c->max_items);
+    A(rc_a32, 2 * N);
+    A(rc_s2n, N);
+    A(rc_list, N);
+    A(rc_cnt, nsys);
+    A(rc_gofs, nsys + 1);
+    A(rc_part, (int64_t)RC_PART_CAP * 2 * RC_GROUP);
+    A(rc_blk, NB);
+    A(rc_bfar, NB);
 #undef A
     HIP_TRY(hipMemcpy(c->sys, c->sys_h.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice));
     std::vector<int32_t> so(N);
@@ -546,6 +575,9 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
         return fail(BH_EINVAL, "slice buffer missing for tail_split (bh_set_tuning failed?)");
     fa.part = c->part;
     fa.part_n = c->N;
+    const bool rc = fp32v && c->rc_kappa > 0.0;
+    fa.a32 = rc ? c->rc_a32 : nullptr;
+    fa.s2n = rc ? c->rc_s2n : nullptr;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
     void* args[] = {&fa};
     unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
@@ -555,9 +587,85 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
         hipLaunchKernelGGL(k_force_combine, dim3(grid_for((int64_t)fa.n_tail * pslots, 256,
                                                           (int64_t)c->num_cu * 32)), dim3(256),
                            0, st, (const Item*)c->items, order, (const int32_t*)c->stats, fa.split,
-                           fa.n_tail, pslots, (const double*)c->part, fa.part_n, p->ax, p->ay);
+                           fa.n_tail, pslots, (const double*)c->part, fa.part_n, p->ax, p->ay,
+                           fa.a32);
     }
+    if (!rc)   // the recomputed count describes the last pass
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_NRECOMP,
+                           (int64_t)2, 0);
     LAUNCH_CHECK("evaluate");
+    c->rc_ready = rc ? 1 : 0;
+    if (rc && !c->rc_deferred) return bh_recompute(ctx, p, law, stream);
+    return BH_OK;
+}
+
+int bh_set_recompute(void* ctx, double kappa, int32_t deferred)
+{
+    Ctx* c = (Ctx*)ctx;
+    if (!c) return fail(BH_EINVAL, "ctx is NULL");
+    if (std::isnan(kappa)) return fail(BH_EINVAL, "kappa is NaN");
+    c->rc_kappa = kappa > 0.0 ? kappa : 0.0;
+    c->rc_deferred = deferred ? 1 : 0;
+    c->rc_ready = 0;
+    return BH_OK;
+}
+
+int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    int r = check_parts(p, "particles");
+    if (r) return r;
+    if (!law) return fail(BH_EINVAL, "law is NULL");
+    if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
+        return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
+    if (!c->rc_ready) return fail(BH_ESTATE, "no FP32 force pass prepared a recompute");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_NRECOMP, (int64_t)2, 0);
+    if (law->G == 0.0) return BH_OK;   // every term is zero
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->S, 64, 1024)), dim3(64), 0, st, c->rc_cnt,
+                       (int64_t)c->S, 0);
+    hipLaunchKernelGGL(k_rc_blocks, dim3(grid_for(c->nblocks, 1, (int64_t)c->num_cu * 16)),
+                       dim3(64), 0, st, (const SysDev*)c->sys, c->S, c->nblocks,
+                       (const double*)c->com_dense, c->rc_blk);
+    hipLaunchKernelGGL(k_rc_blockfar, dim3(grid_for(c->nblocks, 1, (int64_t)c->num_cu * 16)),
+                       dim3(64), 0, st, (const SysDev*)c->sys, c->S, c->nblocks,
+                       (const double*)c->rc_blk, c->rc_bfar);
+    hipLaunchKernelGGL(k_rc_flag, dim3(grid_for(c->max_items, 1, (int64_t)c->num_cu * 16)),
+                       dim3(64), 0, st, (const Item*)c->items, (const SysDev*)c->sys,
+                       (const int32_t*)c->key_sorted, (const int32_t*)c->neprefix,
+                       (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
+                       (const double*)c->rc_blk, (const double*)c->rc_bfar,
+                       (const double*)c->rc_a32,
+                       (const double*)c->rc_s2n, law->G, c->rc_kappa * c->rc_kappa,
+                       (const int32_t*)c->stats, c->rc_cnt, c->rc_list);
+    hipLaunchKernelGGL(k_rc_groups, dim3(1), dim3(256), 0, st, (const int32_t*)c->rc_cnt, c->S,
+                       c->rc_gofs, c->stats);
+    const unsigned g = (unsigned)((int64_t)c->num_cu * 4);
+    if (law->r2_cut > 0.0)
+        hipLaunchKernelGGL(k_rc_fix64<1>, dim3(g), dim3(RC_THREADS), 0, st,
+                           (const int32_t*)c->stats, (const int32_t*)c->rc_cnt,
+                           (const int32_t*)c->rc_gofs, c->S, (const int32_t*)c->rc_list,
+                           (const SysDev*)c->sys, (const int32_t*)c->key_sorted,
+                           (const int32_t*)c->offsets, (const int32_t*)c->neprefix,
+                           (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
+                           (const double*)p->mass, (const double*)c->rc_a32, law->G, law->eps2,
+                           law->r2_cut, c->rc_part, p->ax, p->ay);
+    else
+        hipLaunchKernelGGL(k_rc_fix64<0>, dim3(g), dim3(RC_THREADS), 0, st,
+                           (const int32_t*)c->stats, (const int32_t*)c->rc_cnt,
+                           (const int32_t*)c->rc_gofs, c->S, (const int32_t*)c->rc_list,
+                           (const SysDev*)c->sys, (const int32_t*)c->key_sorted,
+                           (const int32_t*)c->offsets, (const int32_t*)c->neprefix,
+                           (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
+                           (const double*)p->mass, (const double*)c->rc_a32, law->G, law->eps2,
+                           law->r2_cut, c->rc_part, p->ax, p->ay);
+    hipLaunchKernelGGL(k_rc_combine, dim3(grid_for((int64_t)RC_PART_CAP * RC_GROUP, 256, 512)),
+                       dim3(256), 0, st, (const int32_t*)c->stats, (const int32_t*)c->rc_cnt,
+                       (const int32_t*)c->rc_gofs, c->S, (const int32_t*)c->rc_list,
+                       (const SysDev*)c->sys, (const double*)c->rc_part, (const double*)c->rc_a32,
+                       law->G, p->ax, p->ay);
+    LAUNCH_CHECK("recompute");
     return BH_OK;
 }
 
@@ -592,6 +700,8 @@ static int step_impl(void* ctx, const bh_particles* a, const bh_particles* b, co
         ord = order;
     }
     if ((r = bh_force_pass(ctx, b, law, precision, ord, stream))) return r;
+    Ctx* c = (Ctx*)ctx;
+    if (c->rc_ready && c->rc_deferred && (r = bh_recompute(ctx, b, law, stream))) return r;
     if (do_tick && (r = bh_tick_particles(ctx, b, dt, stream))) return r;
     return BH_OK;
 }
@@ -625,6 +735,8 @@ static int step_graph_impl(void* ctx, const bh_particles* a, const bh_particles*
     k.order = order;
     k.part = c->part;
     k.cost_log_on = c->cost_log_on;
+    k.rc_kappa = c->rc_kappa;
+    k.rc_deferred = c->rc_deferred;
     int slot = -1;
     for (int i = 0; i < 2; ++i)
         if (c->gexec[i] && c->gkey[i] == k) slot = i;

@@ -26,20 +26,29 @@ enum {
     ST_QUEUE = 3,      // persistent-kernel queue cursor
     ST_NCELLS_NE = 4,  // number of non-empty cells (all systems)
     ST_NPASS = 5,      // merge passes the within-cell sort needed
+    ST_NRECOMP = 6,    // particles flagged for the FP64 recompute (last FP32 pass)
+    ST_RCGROUPS = 7,   // their groups (up to RC_GROUP particles of one system)
     ST_COUNT = 16
 };
 
-// Per-system parameters. Host fills p_off, n, c_off, col_off, G; the
-// bounding-box kernels fill the FP64 box fields every step.
+// Per-system parameters. Host fills p_off, n, c_off, col_off, b_off, G, Gb;
+// the bounding-box kernels fill the FP64 box fields every step.
 struct SysDev {
     int64_t p_off;
     int64_t n;
     int64_t c_off;
     int64_t col_off;
+    int64_t b_off;   // first of the system's Gb x Gb blocks of RC_BLOCK x RC_BLOCK cells
     int32_t G;
-    int32_t pad;
+    int32_t Gb;
     double min_x, min_y, max_x, max_y, step;
 };
+
+#define RC_BLOCK 16   // cells per side of the blocks the recompute bound aggregates
+#define RC_GROUP 8        // flagged particles evaluated together by one FP64 recompute block
+#define RC_THREADS 256    // threads of that block
+#define RC_SLICES 8       // most COM-list slices per group (when groups are few)
+#define RC_PART_CAP 16384 // (group, slice) partial sums the scratch holds
 
 // Compact list of non-empty cells (one entry per non-empty cell, in global
 // cell order), FP64 physical COM.

@@ -147,16 +147,6 @@ __device__ __forceinline__ bool meets(int b0, int b1, int lo, int hi)
     return lo < hi && b0 < hi && b1 > lo;
 }
 
-// One far-field weight M * w(r2): the same expression on the fast and the
-// masked path, so results do not depend on how particles form items.
-template <int LAW>
-__device__ __forceinline__ double far_w64(double wx, double wy, double M, double eps2, double r2cut)
-{
-    if (LAW == 0) return M / fma(wx, wx, fma(wy, wy, eps2));
-    const double s2 = fma(wx, wx, wy * wy);
-    return (s2 > r2cut) ? M / (s2 + eps2) : 0.0;
-}
-
 template <int LAW>
 __device__ __forceinline__ float far_t32(float wx, float wy, float M, float eps2c, float r2c)
 {
@@ -238,10 +228,7 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
                     for (int k = 0; k < P; ++k) {
                         const bool keep = (f.idx != gi[k]) && (abs(f.row - cy[k]) <= 1);
                         const double wx = f.x - xi[k], wy = f.y - yi[k];
-                        const double s2 = fma(wx, wx, wy * wy);
-                        double w;
-                        if (LAW == 0) w = f.m / (s2 + eps2);
-                        else w = (s2 > r2cut) ? f.m / (s2 + eps2) : 0.0;
+                        double w = near_w64<LAW>(wx, wy, f.m, eps2, r2cut);
                         w = keep ? w : 0.0;
                         sx[k] = fma(wx, w, sx[k]);
                         sy[k] = fma(wy, w, sy[k]);
@@ -398,7 +385,7 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
         const float inv_msc = 1.0f / msc;
 
         double xi[P], yi[P];
-        float ux[P], uy[P], Sx[P], Sy[P], Cx[P], Cy[P], tx[P], ty[P];
+        float ux[P], uy[P], Sx[P], Sy[P], Cx[P], Cy[P], tx[P], ty[P], s2n[P];
         int cy[P], gi[P];
         bool act[P];
 #pragma unroll
@@ -418,10 +405,11 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                 ux[k] = (float)((xi[k] - d.min_x) / step - (double)cx);
                 uy[k] = (float)((yi[k] - d.min_y) / step - (double)oy);
             }
-            Sx[k] = Sy[k] = Cx[k] = Cy[k] = tx[k] = ty[k] = 0.0f;
+            Sx[k] = Sy[k] = Cx[k] = Cy[k] = tx[k] = ty[k] = s2n[k] = 0.0f;
         }
 
-        // near field (separations in FP64, rounded once to FP32); part 0 of a split item
+        // near field (separations in FP64, rounded once to FP32); part 0 of a split item.
+        // s2n sums fm * t = fm^2 / (s2 + eps2c) >= |term|^2 for the recompute's estimate.
         for (int dx = -1; dx <= (part <= 0 ? 1 : -2); ++dx) {
             const int xc = cx + dx;
             if (xc < 0 || xc >= G) continue;
@@ -458,6 +446,7 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                         float t = fm * __builtin_amdgcn_rcpf(s2 + eps2c);
                         if (LAW == 1) t = (s2 > r2c) ? t : 0.0f;
                         t = keep ? t : 0.0f;
+                        s2n[k] = fmaf(fm, t, s2n[k]);
                         if (KAHAN == 2) {
                             kahan_add(Sx[k], Cx[k], wx * t);
                             kahan_add(Sy[k], Cy[k], wy * t);
@@ -478,6 +467,12 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                 }
                 __syncthreads();
             }
+        }
+
+        if (A.s2n && part <= 0) {
+#pragma unroll
+            for (int k = 0; k < P; ++k)
+                if (act[k]) A.s2n[gi[k]] = (double)s2n[k] * ((double)msc * (double)msc);
         }
 
         // far field (a split item's part covers one 64-aligned slice of the list)
@@ -629,6 +624,10 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                 if (part < 0) {
                     A.ax[gi[k]] += scale * rx;
                     A.ay[gi[k]] += scale * ry;
+                    if (A.a32) {
+                        A.a32[2 * (int64_t)gi[k]] = scale * rx;
+                        A.a32[2 * (int64_t)gi[k] + 1] = scale * ry;
+                    }
                 } else {
                     const int64_t o = ((int64_t)part * A.part_n + gi[k]) * 2;
                     A.part[o] = scale * rx;
@@ -648,7 +647,8 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
                                                        const double* __restrict__ part,
                                                        int64_t part_n,
                                                        double* __restrict__ ax,
-                                                       double* __restrict__ ay)
+                                                       double* __restrict__ ay,
+                                                       double* __restrict__ a32)
 {
     const int nitems = stats[ST_NITEMS];
     const int nt = min(n_tail, nitems);
@@ -671,6 +671,10 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
         }
         ax[gi] += sx;
         ay[gi] += sy;
+        if (a32) {
+            a32[2 * gi] = sx;
+            a32[2 * gi + 1] = sy;
+        }
     }
 }
 

@@ -125,6 +125,26 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
                   const int32_t* order, void* stream);
 int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream);
 
+/* FP64 recompute after an FP32 pass (precisions 1-3).  No reference
+ * counterpart: the reference computes in FP64 throughout; this keeps the FP32
+ * path's per-particle error small where a particle's terms nearly cancel.
+ * After the FP32 kernel, every particle gets an upper bound B of
+ * sqrt(sum_j |term_j|^2) (near terms summed exactly, far cells bounded by
+ * their distance to the particle's work item); the FP32 rounding error grows
+ * like eps32 * B, so particles with B > kappa * |a| (a in the law's units
+ * divided by G / step) are evaluated again with the FP64 kernel's terms and
+ * their FP32 result is replaced.  bh_read_stats()[6] is the number of
+ * particles recomputed by the last pass.  kappa <= 0 turns it off (then the
+ * FP32 pass writes no per-particle scratch).  deferred = 0: bh_force_pass,
+ * bh_evaluate_boxes and the step calls run it; deferred = 1: bh_force_pass
+ * stops after the FP32 kernels and the caller runs bh_recompute before
+ * anything else touches the particles (the step calls still run it).
+ * Default: kappa 2, deferred 0. */
+int bh_set_recompute(void* ctx, double kappa, int32_t deferred);
+/* The recompute stage alone, for the last FP32 bh_force_pass on p (same law);
+ * BH_ESTATE if that pass did not prepare it. */
+int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stream);
+
 /* One full round: a is placed and sorted into b, then summarize, evaluate
  * (order NULL: static schedule) and, if do_tick, tick -- all on b. */
 int bh_step(void* ctx, const bh_particles* a, const bh_particles* b, const bh_law* law,
@@ -143,7 +163,8 @@ int bh_step_learned(void* ctx, const bh_particles* a, const bh_particles* b, con
                     int32_t* order, int32_t graph, void* stream);
 
 /* Synchronises the stream, copies the int32 stats block (16 entries:
- * [0] error bits, [1] max cell count, [2] work items, [4] non-empty cells)
+ * [0] error bits, [1] max cell count, [2] work items, [4] non-empty cells,
+ * [6] particles recomputed in FP64 by the last FP32 pass, [7] their groups)
  * to host memory out16, and returns BH_EDEVICE if an error bit is set. */
 int bh_read_stats(void* ctx, int32_t* out16, void* stream);
 /* Copies state `what` into dst (device or host memory), count elements. */

@@ -1,0 +1,223 @@
+"""FP64 recompute after FP32 passes (bh_set_recompute / bh_recompute).
+
+The FP32 kernels' rounding error of a particle grows like eps32 * T with
+T = sqrt(sum_j |term_j|^2); the library bounds T from above per particle and
+recomputes in FP64 the particles with bound > kappa * |a|.  Checked here:
+  * the bound is an upper bound: every particle whose exact T (numpy, FP64,
+    all terms) exceeds kappa * |a| is recomputed;
+  * recomputed particles carry the FP64 path's value (rel. 1e-9: the same
+    terms summed in another order, on particles whose terms cancel);
+  * with kappa tiny every particle is recomputed and the whole pass equals the
+    FP64 pass (one and many systems, cutoff law, softening, with and without
+    COM-list slices);
+  * per-particle error <= 1e-5 on a clustered system where plain FP32 exceeds it;
+  * deferred mode, graph replay and repeated passes give identical results.
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a GPU")]
+
+from barneshut import inputs  # noqa: E402
+
+
+def _engine(n, G, tuning=None):
+    from barneshut.engine import Engine
+    return Engine(n, G, tuning=tuning)
+
+
+def _pass(eng, prec, law, kappa=None, deferred=False):
+    import torch
+    if kappa is not None:
+        eng.ctx.set_recompute(kappa, deferred)
+    eng.parts["ax"].zero_()
+    eng.parts["ay"].zero_()
+    eng.ctx.force_pass(eng.parts, *law, prec)
+    if deferred and prec != "fp64":
+        eng.ctx.recompute(eng.parts, *law)
+    torch.cuda.synchronize()
+    eng.check()
+    return eng.parts["ax"].cpu().numpy().copy(), eng.parts["ay"].cpu().numpy().copy()
+
+
+def _placed(d, n, G, tuning=None):
+    eng = _engine(n, G, tuning)
+    eng.load(d)
+    eng.create_tree()
+    eng.summarize()
+    eng.ctx.build_items()
+    return eng
+
+
+def _rel(ax, ay, rx, ry):
+    return np.hypot(ax - rx, ay - ry) / np.hypot(rx, ry)
+
+
+def _exact_T(h, keys, off, com, G):
+    """sqrt(sum_j |term_j|^2) / G_law per sorted particle, FP64: near particles
+    of the 3x3 block (self excluded) and the COMs of the other non-empty cells,
+    |term| = m / r (no softening)."""
+    n = len(h["px"])
+    x, y, m = h["px"], h["py"], h["mass"]
+    cx, cy = keys // G, keys % G
+    ne = np.nonzero(com[:, 2] != 0)[0]
+    kx, ky = ne // G, ne % G
+    X, Y, M = com[ne, 0], com[ne, 1], com[ne, 2]
+    T2 = np.zeros(n)
+    for a in range(0, n, 2048):
+        b = min(n, a + 2048)
+        dx = X[None, :] - x[a:b, None]
+        dy = Y[None, :] - y[a:b, None]
+        far = (np.abs(kx[None, :] - cx[a:b, None]) > 1) | (np.abs(ky[None, :] - cy[a:b, None]) > 1)
+        r2 = np.where(far, dx * dx + dy * dy, 1.0)
+        T2[a:b] = np.where(far, M[None, :] ** 2 / r2, 0.0).sum(1)
+    for c in np.nonzero(np.diff(off))[0]:
+        gx, gy = c // G, c % G
+        own = np.arange(off[c], off[c + 1])
+        nb = [np.arange(off[u * G + v], off[u * G + v + 1])
+              for u in range(max(gx - 1, 0), min(gx + 2, G)) for v in range(max(gy - 1, 0), min(gy + 2, G))]
+        nb = np.concatenate(nb)
+        dx = x[nb][None, :] - x[own][:, None]
+        dy = y[nb][None, :] - y[own][:, None]
+        r2 = dx * dx + dy * dy
+        self_ = nb[None, :] == own[:, None]
+        T2[own] += np.where(self_, 0.0, m[nb][None, :] ** 2 / np.where(self_, 1.0, r2)).sum(1)
+    return np.sqrt(T2)
+
+
+def test_bound_covers_every_particle_over_kappa():
+    """Every particle with exact T > kappa |a| / G is recomputed (so the
+    library's estimate is no smaller than T), and every recomputed particle
+    holds the FP64 value."""
+    n, G, law = 20000, 32, (1.0, 0.0, 0.0)
+    d = inputs.gaussian_mixture(n, 4, (0.02, 0.1), seed=310)   # no coincident particles
+    eng = _placed(d, [n], [G])
+    r64 = _pass(eng, "fp64", law)
+    r32 = _pass(eng, "fp32", law, kappa=0.0)
+    kappa = 0.5
+    rrc = _pass(eng, "fp32", law, kappa=kappa)
+    nrc = int(eng.ctx.stats()[6])
+    h = eng.fetch()
+    off = eng.offsets().astype(np.int64)
+    keys = eng.keys().astype(np.int64)
+    com = eng.com()
+    eng.close()
+    T = _exact_T(h, keys, off, com, G)
+    amag = np.hypot(r64[0], r64[1])
+    must = T > kappa * amag * (1 + 1e-6)
+    changed = (rrc[0] != r32[0]) | (rrc[1] != r32[1])
+    print(f"[bound] must {must.sum()} recomputed {nrc} changed {changed.sum()}", flush=True)
+    assert must.sum() > 0
+    assert changed.sum() <= nrc
+    # the bound holds: no particle over kappa escapes the recompute
+    assert np.all(changed[must])
+    # recomputed particles carry the FP64 path's value (up to summation order)
+    err = _rel(rrc[0], rrc[1], r64[0], r64[1])
+    assert err[changed].max() < 1e-9
+    # and the bound is not wildly loose: a bounded multiple of the necessary set
+    assert nrc <= 20 * must.sum() + 100
+
+
+@pytest.mark.parametrize("case", ["one_system_slices", "many_systems_cutoff_soft", "all_groups"])
+def test_tiny_kappa_recomputes_everything(case):
+    """kappa ~ 0: every particle is recomputed and the pass equals the FP64 pass."""
+    if case == "one_system_slices":                  # few groups: COM-list slices
+        n, G, law = [3000], [24], (1.3, 0.0, 0.0)
+        d = inputs.gaussian_mixture(3000, 3, (0.02, 0.1), seed=302)
+    elif case == "many_systems_cutoff_soft":         # LAW 1, softening, 7 systems
+        sizes = np.array([900, 1500, 64, 2500, 700, 1200, 333])
+        n, G, law = list(sizes), [8, 10, 2, 12, 7, 9, 5], (0.7, 1e-6, 1e-5)
+        d = inputs.batched_uniform(sizes, seed=303)
+    else:                                            # > 16384 / 8 groups: no slices
+        n, G, law = [140000], [96], (1.0, 0.0, 0.0)
+        d = inputs.gaussian_mixture(140000, 8, (0.01, 0.1), seed=310)
+    eng = _placed(d, n, G)
+    r64 = _pass(eng, "fp64", law)
+    rrc = _pass(eng, "fp32", law, kappa=1e-30)
+    nrc = int(eng.ctx.stats()[6])
+    eng.close()
+    err = _rel(rrc[0], rrc[1], r64[0], r64[1])
+    print(f"[{case}] recomputed {nrc} of {sum(n)}, max rel vs fp64 {np.nanmax(err):.3e}", flush=True)
+    assert nrc == sum(n)
+    assert np.isfinite(err).all() and err.max() < 1e-9
+
+
+def test_per_particle_error_meets_target_on_clusters():
+    """A clustered system on a fine grid: plain FP32 has particles over 1e-5
+    relative error; with the recompute (default kappa 2) none has."""
+    n, G, law = 1 << 18, 256, (1e-6, 0.0, 0.0)
+    d = inputs.gaussian_mixture(n, 16, (0.001, 0.03), seed=305, background=0.2)
+    eng = _placed(d, [n], [G])
+    r64 = _pass(eng, "fp64", law)
+    r32 = _pass(eng, "fp32", law, kappa=0.0)
+    rrc = _pass(eng, "fp32", law, kappa=2.0)
+    nrc = int(eng.ctx.stats()[6])
+    eng.close()
+    e32 = _rel(r32[0], r32[1], r64[0], r64[1])
+    erc = _rel(rrc[0], rrc[1], r64[0], r64[1])
+    print(f"[target] fp32 max {e32.max():.3e} (>1e-5: {(e32 > 1e-5).sum()}), with recompute "
+          f"max {erc.max():.3e}, recomputed {nrc}", flush=True)
+    assert erc.max() <= 1e-5
+    assert erc.max() <= e32.max()
+    # 2301 particles (0.9 %) have an exact T / |a| over 2 here; the bound flags
+    # about 2.8x that
+    assert 0 < nrc < n // 20
+
+
+def test_deferred_graph_and_repeat_identical():
+    import torch
+    n, G, law = 30000, 40, (1.0, 0.0, 0.0)
+    d = inputs.gaussian_mixture(n, 5, (0.01, 0.1), seed=310)
+    eng = _placed(d, [n], [G])
+    a = _pass(eng, "fp32", law, kappa=2.0)
+    b = _pass(eng, "fp32", law, kappa=2.0, deferred=True)
+    c = _pass(eng, "fp32", law, kappa=2.0)
+    assert int(eng.ctx.stats()[6]) > 0
+    for u, v in ((a, b), (a, c)):
+        np.testing.assert_array_equal(u[0], v[0])
+        np.testing.assert_array_equal(u[1], v[1])
+    # recompute without a prepared FP32 pass is refused
+    _pass(eng, "fp64", law, kappa=2.0)
+    from barneshut.kernels.hiplib import BhError
+    with pytest.raises(BhError):
+        eng.ctx.recompute(eng.parts, *law)
+    eng.close()
+
+    # whole steps: graph replays (recompute inside) equal eager steps
+    res = []
+    for graph in (False, True):
+        e = _engine([n], [G])
+        e.ctx.set_recompute(2.0)
+        e.load(d)
+        out = []
+        for _ in range(3):
+            e.step(*law, precision="fp32", dt=1e-5, graph=graph)
+            torch.cuda.synchronize()
+            out.append({k: v.copy() for k, v in e.fetch().items()})
+            out[-1]["nrc"] = int(e.ctx.stats()[6])
+        e.check()
+        e.close()
+        res.append(out)
+    for s in range(3):
+        assert res[0][s]["nrc"] == res[1][s]["nrc"] > 0
+        for k in ("px", "py", "vx", "vy", "pid"):
+            np.testing.assert_array_equal(res[0][s][k], res[1][s][k])
+
+
+def test_recompute_off_leaves_fp32_pass_unchanged():
+    """kappa 0 turns the stage off: nothing recomputed, stats slot 0, and the
+    FP32 result is the one the FP32 kernels give with the stage on for the
+    particles it did not flag."""
+    n, G, law = 20000, 32, (1.0, 0.0, 0.0)
+    d = inputs.gaussian_mixture(n, 4, (0.02, 0.1), seed=307)
+    eng = _placed(d, [n], [G])
+    on = _pass(eng, "fp32", law, kappa=2.0)
+    assert int(eng.ctx.stats()[6]) > 0
+    off = _pass(eng, "fp32", law, kappa=0.0)
+    assert int(eng.ctx.stats()[6]) == 0
+    same = (on[0] == off[0]) & (on[1] == off[1])
+    assert same.sum() >= n - 1000
+    eng.close()

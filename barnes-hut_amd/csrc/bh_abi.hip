@@ -95,6 +95,7 @@ struct bh_ctx_s {
     int32_t* rc_cnt = nullptr;     // per system
     int32_t* rc_gofs = nullptr;    // per system: first group (S + 1)
     double* rc_part = nullptr;     // (group, slice) partial sums
+    double* rc_tiles = nullptr;    // per RC_TILE^2 tile: sum of M^2
     double* rc_blk = nullptr;      // per block: sum of M^2
     double* rc_bfar = nullptr;     // per block: its far-block part of the bound
     int64_t nblocks = 0;
@@ -260,6 +261,7 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     A(rc_cnt, nsys);
     A(rc_gofs, nsys + 1);
     A(rc_part, (int64_t)RC_PART_CAP * 2 * RC_GROUP);
+    A(rc_tiles, NB * (RC_BLOCK / RC_TILE) * (RC_BLOCK / RC_TILE));
     A(rc_blk, NB);
     A(rc_bfar, NB);
 #undef A
@@ -628,13 +630,18 @@ int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stre
     hipLaunchKernelGGL(k_rc_blocks, dim3(grid_for(c->nblocks, 1, (int64_t)c->num_cu * 16)),
                        dim3(64), 0, st, (const SysDev*)c->sys, c->S, c->nblocks,
                        (const double*)c->com_dense, c->rc_blk);
+    const int64_t ntiles = c->nblocks * (RC_BLOCK / RC_TILE) * (RC_BLOCK / RC_TILE);
+    hipLaunchKernelGGL(k_rc_tiles, dim3(grid_for(ntiles, 256, (int64_t)c->num_cu * 8)), dim3(256),
+                       0, st, (const SysDev*)c->sys, c->S, ntiles, (const double*)c->com_dense,
+                       c->rc_tiles);
     hipLaunchKernelGGL(k_rc_blockfar, dim3(grid_for(c->nblocks, 1, (int64_t)c->num_cu * 16)),
                        dim3(64), 0, st, (const SysDev*)c->sys, c->S, c->nblocks,
                        (const double*)c->rc_blk, c->rc_bfar);
-    hipLaunchKernelGGL(k_rc_flag, dim3(grid_for(c->max_items, 1, (int64_t)c->num_cu * 16)),
+    hipLaunchKernelGGL(k_rc_flag, dim3(grid_for(c->max_items, 1, (int64_t)c->num_cu * 32)),
                        dim3(64), 0, st, (const Item*)c->items, (const SysDev*)c->sys,
                        (const int32_t*)c->key_sorted, (const int32_t*)c->neprefix,
                        (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
+                       (const double*)c->com_dense, (const double*)c->rc_tiles,
                        (const double*)c->rc_blk, (const double*)c->rc_bfar,
                        (const double*)c->rc_a32,
                        (const double*)c->rc_s2n, law->G, c->rc_kappa * c->rc_kappa,

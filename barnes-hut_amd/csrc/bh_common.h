@@ -45,6 +45,8 @@ struct SysDev {
 };
 
 #define RC_BLOCK 16   // cells per side of the blocks the recompute bound aggregates
+#define RC_EXACT 3    // cells around a work item whose terms the bound takes exactly
+#define RC_TILE 4     // cells per side of the bound's middle level (divides RC_BLOCK)
 #define RC_GROUP 8        // flagged particles evaluated together by one FP64 recompute block
 #define RC_THREADS 256    // threads of that block
 #define RC_SLICES 8       // most COM-list slices per group (when groups are few)

@@ -117,11 +117,14 @@ __global__ void k_force_combine(const Item* items, const int32_t* order, const i
 // large against their acceleration
 __global__ void k_rc_blocks(const SysDev* sys, int S, int64_t nblocks, const double* com_dense,
                             double* blk);
+__global__ void k_rc_tiles(const SysDev* sys, int S, int64_t ntiles, const double* com_dense,
+                           double* tl);
 __global__ void k_rc_blockfar(const SysDev* sys, int S, int64_t nblocks, const double* blk,
                               double* bfar);
 __global__ void k_rc_flag(const Item* items, const SysDev* sys, const int32_t* key_sorted,
                           const int32_t* neprefix, const Com64* l64, const double* x,
-                          const double* y, const double* blk, const double* bfar,
+                          const double* y, const double* com, const double* tl,
+                          const double* blk, const double* bfar,
                           const double* a32, const double* s2n,
                           double Gc, double kappa2, const int32_t* stats, int32_t* rc_cnt,
                           int32_t* list);

@@ -8,18 +8,24 @@
 // relative error is large even though the normwise error over all particles
 // stays ~1e-7.  After an FP32 pass, k_rc_flag bounds sqrt(sum_j |term_j|^2)
 // from above for every particle and flags those whose bound exceeds kappa
-// times their acceleration; k_rc_fix64 evaluates the flagged particles with
-// the FP64 kernel's terms and replaces their FP32 result.
+// times their acceleration; k_rc_fix64 evaluates the flagged particles in
+// FP64 (the law's terms with a Newton-refined reciprocal) and replaces their
+// FP32 result.
 //
 // The bound, for a work item (particles of column cx, rows [y0, y1]; box
 // [cx, cx + 1] x [y0, y1 + 1] in cell units):
 //  * near terms: summed exactly by the FP32 kernel (s2n, one fma per term);
-//  * far cells of the masked region (columns cx-1..cx+1, rows y0-1..y1+1)
-//    that are far for the particle: exact (M / d)^2, per particle;
-//  * other non-empty cells of the item's window (the RC_BLOCK x RC_BLOCK
-//    blocks within one block of the item's blocks): M^2 / max(1, dist(box,
-//    COM))^2 -- a far cell's COM is at least one cell from each of the item's
-//    particles, and at least dist(box, COM) away;
+//  * far cells of the exact region (columns cx-R..cx+R, rows y0-R..y1+R,
+//    R = RC_EXACT) that are far for the particle: exact (M / d)^2, per
+//    particle -- computed only for particles that a cheaper bound of this
+//    region (M^2 / max(1, dist(box, COM))^2 per cell, taken per item) does
+//    not already clear;
+//  * other cells of the exact region's hull in RC_TILE x RC_TILE tiles:
+//    M^2 / max(1, dist(box, COM))^2 -- each of the item's particles is at
+//    least that far from the COM;
+//  * other tiles of the item's window (the RC_BLOCK x RC_BLOCK blocks within
+//    one block of the item's blocks): (sum of M^2 over the tile) / dist(box,
+//    tile)^2, with dist >= R;
 //  * blocks outside that window: (sum of M^2 over the block) / dist(box,
 //    block)^2, with dist >= RC_BLOCK; for an item inside one block this part
 //    is taken per block, with the distance from the block's box (no shorter
@@ -68,6 +74,41 @@ __global__ __launch_bounds__(64) void k_rc_blocks(const SysDev* __restrict__ sys
     }
 }
 
+// Sum of M^2 over each RC_TILE x RC_TILE tile of cells (one thread per tile,
+// fixed order); system s's tiles start at b_off * (RC_BLOCK / RC_TILE)^2, in
+// a Gb * RC_BLOCK / RC_TILE square.
+__global__ __launch_bounds__(256) void k_rc_tiles(const SysDev* __restrict__ sys, int S,
+                                                  int64_t ntiles,
+                                                  const double* __restrict__ com_dense,
+                                                  double* __restrict__ tl)
+{
+    constexpr int TPB = RC_BLOCK / RC_TILE;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < ntiles; q += stride) {
+        int lo = 0, hi = S - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (sys[mid].b_off * TPB * TPB <= q) lo = mid; else hi = mid - 1;
+        }
+        const SysDev& d = sys[lo];
+        const int Gt = d.Gb * TPB;
+        const int64_t lt = q - d.b_off * TPB * TPB;
+        const int tx = (int)(lt / Gt), ty = (int)(lt - (int64_t)tx * Gt);
+        double acc = 0.0;
+        for (int i = 0; i < RC_TILE; ++i) {
+            const int gx = tx * RC_TILE + i;
+            for (int j = 0; j < RC_TILE; ++j) {
+                const int gy = ty * RC_TILE + j;
+                if (gx < d.G && gy < d.G) {
+                    const double M = com_dense[3 * (d.c_off + (int64_t)gx * d.G + gy) + 2];
+                    acc = fma(M, M, acc);
+                }
+            }
+        }
+        tl[q] = acc;
+    }
+}
+
 // Per block b: sum over the blocks outside b's 3 x 3 block neighbourhood of
 // (sum of M^2) / dist(b, b')^2 (one wave per block, fixed order).
 __global__ __launch_bounds__(64) void k_rc_blockfar(const SysDev* __restrict__ sys, int S,
@@ -112,6 +153,8 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
                                                 const Com64* __restrict__ l64,
                                                 const double* __restrict__ x,
                                                 const double* __restrict__ y,
+                                                const double* __restrict__ com,
+                                                const double* __restrict__ tl,
                                                 const double* __restrict__ blk,
                                                 const double* __restrict__ bfar,
                                                 const double* __restrict__ a32,
@@ -120,11 +163,10 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
                                                 int32_t* rc_cnt, int32_t* __restrict__ list)
 {
     __shared__ Com64 tile[BH_WAVE];
-    __shared__ int32_t colpre[BH_WAVE], colbeg[BH_WAVE];
     const int lane = threadIdx.x;
     const int nitems = stats[ST_NITEMS];
     int psys = -1, pcx = -1, py0 = -1, py1 = -1;   // box of the last bound (wave-uniform)
-    double psi = 0.0;
+    double psi = 0.0, preg = 0.0;   // item bound outside / over the exact region
     for (int itx = blockIdx.x; itx < nitems; itx += gridDim.x) {
         const Item it = items[itx];
         const SysDev& d = sys[it.sys];
@@ -135,48 +177,49 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
         const double bx0 = cx, bx1 = cx + 1, by0 = y0, by1 = y1 + 1;
         const int wbx0 = max(cx / RC_BLOCK - 1, 0), wbx1 = min(cx / RC_BLOCK + 1, Gb - 1);
         const int wby0 = max(y0 / RC_BLOCK - 1, 0), wby1 = min(y1 / RC_BLOCK + 1, Gb - 1);
-        const int wx0 = wbx0 * RC_BLOCK, wx1 = min(G, (wbx1 + 1) * RC_BLOCK);
-        const int wy0 = wby0 * RC_BLOCK, wy1 = min(G, (wby1 + 1) * RC_BLOCK);
-        const int ncolw = wx1 - wx0;   // <= 3 * RC_BLOCK <= 64
         const bool same = it.sys == psys && cx == pcx && y0 == py0 && y1 == py1;
         psys = it.sys;
         pcx = cx;
         py0 = y0;
         py1 = y1;
         if (!same) {
-            // non-empty cells of the window outside the masked region: lane l
-            // finds column wx0 + l's entries, then the lanes walk them flattened
-            int cb = 0, cn = 0;
-            if (lane < ncolw) {
-                const int64_t cl = coff + (int64_t)(wx0 + lane) * G;
-                cb = neprefix[cl + wy0];
-                cn = neprefix[cl + wy1] - cb;
-            }
-            int inc = cn;
-#pragma unroll
-            for (int o = 1; o < BH_WAVE; o <<= 1) {
-                const int v = __shfl_up(inc, o, BH_WAVE);
-                if (lane >= o) inc += v;
-            }
-            const int tot = __shfl(inc, BH_WAVE - 1, BH_WAVE);
-            colpre[lane] = inc - cn;
-            colbeg[lane] = cb;
-            __syncthreads();
+            // cells of the exact region's hull in RC_TILE x RC_TILE tiles, outside
+            // the exact region (dense walk)
+            const int htx0 = max(cx - RC_EXACT, 0) / RC_TILE, htx1 = min(cx + RC_EXACT, G - 1) / RC_TILE;
+            const int hty0 = max(y0 - RC_EXACT, 0) / RC_TILE, hty1 = min(y1 + RC_EXACT, G - 1) / RC_TILE;
+            const int hx0 = htx0 * RC_TILE, hx1 = min(G, (htx1 + 1) * RC_TILE);
+            const int hy0 = hty0 * RC_TILE, hy1 = min(G, (hty1 + 1) * RC_TILE);
+            const int hrows = hy1 - hy0, nh = (hx1 - hx0) * hrows;
             psi = 0.0;
-            for (int t = lane; t < tot; t += BH_WAVE) {
-                int lo = 0, hi = ncolw - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (colpre[mid] <= t) lo = mid; else hi = mid - 1;
-                }
-                const Com64 f = l64[colbeg[lo] + (t - colpre[lo])];
-                if (abs(f.kx - cx) <= 1 && f.ky >= y0 - 1 && f.ky <= y1 + 1) continue;
-                const double X = (f.X - d.min_x) * inv_step, Y = (f.Y - d.min_y) * inv_step;
+            for (int t = lane; t < nh; t += BH_WAVE) {
+                const int col = t / hrows;
+                const int gx = hx0 + col, gy = hy0 + (t - col * hrows);
+                if (abs(gx - cx) <= RC_EXACT && gy >= y0 - RC_EXACT && gy <= y1 + RC_EXACT) continue;
+                const double* cm = com + 3 * (coff + (int64_t)gx * G + gy);
+                const double M = cm[2];
+                if (M == 0.0) continue;
+                const double X = (cm[0] - d.min_x) * inv_step, Y = (cm[1] - d.min_y) * inv_step;
                 const double ex = fmax(fmax(bx0 - X, X - bx1), 0.0);
                 const double ey = fmax(fmax(by0 - Y, Y - by1), 0.0);
-                psi = fma(f.M * f.M, __builtin_amdgcn_rcp(fmax(fma(ex, ex, ey * ey), 1.0)), psi);
+                psi = fma(M * M, __builtin_amdgcn_rcp(fmax(fma(ex, ex, ey * ey), 1.0)), psi);
             }
-            __syncthreads();
+            // tiles of the window outside the hull (at least RC_EXACT cells away)
+            constexpr int TPB = RC_BLOCK / RC_TILE;
+            const int Gt = Gb * TPB;
+            const int tx0 = wbx0 * TPB, ntx = (wbx1 - wbx0 + 1) * TPB;
+            const int ty0 = wby0 * TPB, nty = (wby1 - wby0 + 1) * TPB;
+            const int64_t toff = d.b_off * TPB * TPB;
+            for (int t = lane; t < ntx * nty; t += BH_WAVE) {
+                const int a = t / nty;
+                const int tx = tx0 + a, ty = ty0 + (t - a * nty);
+                if (tx >= htx0 && tx <= htx1 && ty >= hty0 && ty <= hty1) continue;
+                const double T2 = tl[toff + (int64_t)tx * Gt + ty];
+                if (T2 == 0.0) continue;
+                const double lx = (double)tx * RC_TILE, ly = (double)ty * RC_TILE;
+                const double ex = fmax(fmax(lx - bx1, bx0 - (lx + RC_TILE)), 0.0);
+                const double ey = fmax(fmax(ly - by1, by0 - (ly + RC_TILE)), 0.0);
+                psi = fma(T2, __builtin_amdgcn_rcp(fma(ex, ex, ey * ey)), psi);
+            }
             psi = wave_sum(psi);
             // blocks outside the window
             if (y0 / RC_BLOCK == y1 / RC_BLOCK) {
@@ -197,41 +240,86 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
                 }
                 psi += wave_sum(pb);
             }
+            // the exact region at item level, for the cheap first test:
+            // M^2 / max(1, dist(box, COM))^2; a one-row item's 3 x 3 block is
+            // near for all of its particles and left out
+            const int rxa = max(cx - RC_EXACT, 0), rxb = min(cx + RC_EXACT, G - 1);
+            const int rya = max(y0 - RC_EXACT, 0), ryb = min(y1 + RC_EXACT, G - 1);
+            const int rrows = ryb - rya + 1, nr = (rxb - rxa + 1) * rrows;
+            preg = 0.0;
+            for (int t = lane; t < nr; t += BH_WAVE) {
+                const int col = t / rrows;
+                const int gx = rxa + col, gy = rya + (t - col * rrows);
+                if (y0 == y1 && abs(gx - cx) <= 1 && abs(gy - y0) <= 1) continue;
+                const double* cm = com + 3 * (coff + (int64_t)gx * G + gy);
+                const double M = cm[2];
+                if (M == 0.0) continue;
+                const double X = (cm[0] - d.min_x) * inv_step, Y = (cm[1] - d.min_y) * inv_step;
+                const double ex = fmax(fmax(bx0 - X, X - bx1), 0.0);
+                const double ey = fmax(fmax(by0 - Y, Y - by1), 0.0);
+                preg = fma(M * M, __builtin_amdgcn_rcp(fmax(fma(ex, ex, ey * ey), 1.0)), preg);
+            }
+            preg = wave_sum(preg);
         }
 
         // |a| in the bound's units (mass / cells): a * step / G
         const double g2 = (d.step / Gc) * (d.step / Gc);
-        const int xa = max(cx - 1, 0), xb = min(cx + 1, G - 1);
-        const int ya = max(y0 - 1, 0), yb = min(y1 + 1, G - 1);
+        // the exact region's entries: one compact range per column (lane c <
+        // 2 * RC_EXACT + 1 holds column xa + c), walked as one flattened list
+        const int xa = max(cx - RC_EXACT, 0), xb = min(cx + RC_EXACT, G - 1);
+        const int ya = max(y0 - RC_EXACT, 0), yb = min(y1 + RC_EXACT, G - 1);
+        const int nxc = xb - xa + 1;
+        int eb = 0, en = 0;
+        if (lane < nxc) {
+            const int64_t cl = coff + (int64_t)(xa + lane) * G;
+            eb = neprefix[cl + ya];
+            en = neprefix[cl + yb + 1] - eb;
+        }
+        int einc = en;
+#pragma unroll
+        for (int o = 1; o < 2 * RC_EXACT + 1; o <<= 1) {
+            const int v = __shfl_up(einc, o, BH_WAVE);
+            if (lane >= o) einc += v;
+        }
+        const int etot = __shfl(einc, nxc - 1, BH_WAVE);
+        const int epre = einc - en;
         for (int base = 0; base < it.np; base += BH_WAVE) {
             const int slot = base + lane;
             const bool act = slot < it.np;
             const int64_t gi = (int64_t)it.p0 + (act ? slot : 0);
             const double xi = x[gi], yi = y[gi];
             const int cy = (int)(key_sorted[gi] - coff) - cx * G;
-            double est = psi + s2n[gi];
+            const double sn = s2n[gi];
             const double axv = a32[2 * gi], ayv = a32[2 * gi + 1];
-            // far cells of the masked region, exact per particle (LDS tiles)
-            for (int gx = xa; gx <= xb; ++gx) {
-                const int64_t cl = coff + (int64_t)gx * G;
-                const int m0 = neprefix[cl + ya], m1 = neprefix[cl + yb + 1];
-                for (int tb = m0; tb < m1; tb += BH_WAVE) {
-                    if (tb + lane < m1) tile[lane] = l64[tb + lane];
-                    __syncthreads();
-                    const int cnt = min(BH_WAVE, m1 - tb);
-                    for (int j = 0; j < cnt; ++j) {
-                        const Com64 f = tile[j];
-                        if (abs(f.ky - cy) >= 2) {
-                            const double ddx = (f.X - xi) * inv_step, ddy = (f.Y - yi) * inv_step;
-                            est = fma(f.M * f.M, __builtin_amdgcn_rcp(fma(ddx, ddx, ddy * ddy)), est);
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
             const double A2 = fma(axv, axv, ayv * ayv) * g2;
+            // first test with the item-level bound of the exact region; the
+            // exact per-particle sum only runs where some lane passes it
+            const double loose = psi + preg + sn;
+            const bool cand = act && loose != 0.0 && !(loose <= kappa2 * A2);
+            if (__ballot(cand) == 0ull) continue;
+            double est = psi + sn;
+            // far cells of the exact region, per particle (LDS tiles)
+            for (int tb = 0; tb < etot; tb += BH_WAVE) {
+                const int t = tb + lane;
+                int c = 0;
+#pragma unroll
+                for (int k = 1; k < 2 * RC_EXACT + 1; ++k)
+                    if (k < nxc && __shfl(epre, k, BH_WAVE) <= t) c = k;
+                const int e = __shfl(eb, c, BH_WAVE) + (t - __shfl(epre, c, BH_WAVE));
+                if (t < etot) tile[lane] = l64[e];
+                __syncthreads();
+                const int cnt = min(BH_WAVE, etot - tb);
+                for (int j = 0; j < cnt; ++j) {
+                    const Com64 f = tile[j];
+                    if (max(abs(f.kx - cx), abs(f.ky - cy)) >= 2) {
+                        const double ddx = (f.X - xi) * inv_step, ddy = (f.Y - yi) * inv_step;
+                        est = fma(f.M * f.M, __builtin_amdgcn_rcp(fma(ddx, ddx, ddy * ddy)), est);
+                    }
+                }
+                __syncthreads();
+            }
             // NaN or infinite estimates flag too: the FP64 terms decide
-            const bool flag = act && est != 0.0 && !(est <= kappa2 * A2);
+            const bool flag = cand && est != 0.0 && !(est <= kappa2 * A2);
             const unsigned long long mask = __ballot(flag);
             const int n = __popcll(mask);
             if (n == 0) continue;
@@ -275,6 +363,24 @@ __global__ __launch_bounds__(256) void k_rc_groups(const int32_t* __restrict__ r
         stats[ST_NRECOMP] = carry_n;
         stats[ST_RCGROUPS] = carry_g;
     }
+}
+
+// 1 / q to FP64 accuracy: v_rcp_f64 refined by two Newton steps.
+__device__ __forceinline__ double rcp64(double q)
+{
+    double r = __builtin_amdgcn_rcp(q);
+    double e = fma(-q, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-q, r, 1.0);
+    return fma(r, e, r);
+}
+
+template <int LAW>
+__device__ __forceinline__ double rc_w64(double wx, double wy, double m, double eps2, double r2cut)
+{
+    const double s2 = fma(wx, wx, wy * wy);
+    if (LAW == 0) return m * rcp64(s2 + eps2);
+    return (s2 > r2cut) ? m * rcp64(s2 + eps2) : 0.0;
 }
 
 __device__ __forceinline__ int rc_slices(int ngroups)
@@ -347,7 +453,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_rc_fix64(
             for (int k = 0; k < RC_GROUP; ++k) {
                 const bool keep = max((unsigned)(f.kx - cxk[k] + 1), (unsigned)(f.ky - cyk[k] + 1)) > 2u;
                 const double wx = f.X - xk[k], wy = f.Y - yk[k];
-                double w = far_w64<LAW>(wx, wy, f.M, eps2, r2cut);
+                double w = rc_w64<LAW>(wx, wy, f.M, eps2, r2cut);
                 w = keep ? w : 0.0;
                 sx[k] = fma(wx, w, sx[k]);
                 sy[k] = fma(wy, w, sy[k]);
@@ -364,7 +470,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_rc_fix64(
                         const int pa = offsets[cl + ra], pb = offsets[cl + rb + 1];
                         for (int j = pa + tid; j < pb; j += RC_THREADS) {
                             const double wx = x[j] - xk[k], wy = y[j] - yk[k];
-                            double w = near_w64<LAW>(wx, wy, m[j], eps2, r2cut);
+                            double w = rc_w64<LAW>(wx, wy, m[j], eps2, r2cut);
                             w = (j != gk[k]) ? w : 0.0;
                             sx[k] = fma(wx, w, sx[k]);
                             sy[k] = fma(wy, w, sy[k]);

@@ -12,7 +12,10 @@ per-cell COM, forces, tick.  Paths on the same input:
   * FP32 cell-relative forces, Kahan-compensated tile sums, static order;
   * the same FP32 path with the learned work-queue order (the cost model in
     barnes-hut_amd/costmodels/clusters256.npz; --cost-model none disables).
-`value` is the FP32 + learned-order path's whole-job throughput.
+The FP32 paths include the FP64 recompute of the particles whose FP32 terms
+nearly cancel (kappa 2, --recompute-kappa 0 disables); it is timed apart from
+the force kernel.  `value` is the FP32 + learned-order path's whole-job
+throughput.
 
 Secondary lines (one untimed-in-`value` measurement each): 512 independent
 systems stepped by one graph launch per step (static vs learned order), and
@@ -57,17 +60,24 @@ TRAFFIC = 3.45e9
 # interaction (v_rcp_f32 co-issued), no packed FP32, 256 CUs x 4 SIMDs x 16
 # lanes x 2.4 GHz = 39.3e12 lane-instructions/s -> 6.55e12 interactions/s.
 ISSUE_CEILING_TFLOPS = 256 * 4 * 16 * 2.4e9 / 6 * FLOP_PER_INTERACTION / 1e12
+# FP64 recompute threshold of the FP32 paths (--recompute-kappa)
+RECOMPUTE_KAPPA = 2.0
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_path(eng, precision, steps, warmup, order_fn=None):
+def run_path(eng, precision, steps, warmup, order_fn=None, kappa=RECOMPUTE_KAPPA):
     """Warm-up, then `steps` timed full steps.  Returns (seconds per step,
-    mean force-kernel ms, interactions of the last step)."""
+    force-kernel ms per step, recompute ms per step, near and far interactions
+    per step).  The FP32 force pass runs deferred, so that HIP events split the
+    force kernels (with the slice combine) from the FP64 recompute stage."""
     import torch
     ctx = eng.ctx
+    fp32 = precision != "fp64"
+    if fp32:
+        ctx.set_recompute(kappa, deferred=True)
 
     n_tot = int(eng.n.sum())
     counts = torch.zeros((steps, 4), dtype=torch.int64, device=eng.dev)
@@ -84,31 +94,36 @@ def run_path(eng, precision, steps, warmup, order_fn=None):
         ctx.force_pass(eng.parts, GC, EPS2, R2CUT, precision, order)
         if events is not None:
             events[1].record()
+        if fp32 and kappa > 0:
+            ctx.recompute(eng.parts, GC, EPS2, R2CUT)
+        if events is not None:
+            events[2].record()
         eng.timestep(DT)
 
     for _ in range(warmup):
         one()
     eng.check()
     torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
+    evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(steps)]
     t0 = time.perf_counter()
     for s in range(steps):
         one(evs[s], s)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     eng.check()
-    kms = np.array([a.elapsed_time(b) for a, b in evs])
+    kms = np.array([a.elapsed_time(b) for a, b, _ in evs])
+    rms = np.array([b.elapsed_time(c) for _, b, c in evs])
     c = counts.cpu().numpy()
     near = c[:, 2] - n_tot
     far = c[:, 3]
-    return (t1 - t0) / steps, kms, near, far
+    return (t1 - t0) / steps, kms, rms, near, far
 
 
-def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7):
+def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7, kappa=RECOMPUTE_KAPPA):
     """FP32 vs FP64 accelerations on the same placement: normwise, p99 and
     max per-particle relative error over all N particles (FP64 GPU path as
-    the reference), plus FP64 GPU path vs the CPU oracle on a sample."""
+    the reference), with and without the FP64 recompute, plus FP64 GPU path vs
+    the CPU oracle on a sample."""
     import torch
     eng = eng_cls([len(d["px"])], [G])
     eng.load(d)
@@ -118,15 +133,23 @@ def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7):
     eng.ctx.force_pass(eng.parts, GC, EPS2, R2CUT, "fp64")
     a64 = (eng.parts["ax"].clone(), eng.parts["ay"].clone())
     out = {}
-    for prec in ("fp32", "fp32_kahan_term", "fp32_plain"):
+    runs = [("fp32", "fp32", kappa), ("fp32_no_recompute", "fp32", 0.0),
+            ("fp32_kahan_term", "fp32_kahan_term", kappa),
+            ("fp32_kahan_term_no_recompute", "fp32_kahan_term", 0.0),
+            ("fp32_plain_no_recompute", "fp32_plain", 0.0)]
+    for name, prec, kap in runs:
+        eng.ctx.set_recompute(kap)
         eng.parts["ax"].zero_()
         eng.parts["ay"].zero_()
         eng.ctx.force_pass(eng.parts, GC, EPS2, R2CUT, prec)
         num = torch.hypot(eng.parts["ax"] - a64[0], eng.parts["ay"] - a64[1])
         den = torch.hypot(a64[0], a64[1])
         per = (num / den).cpu().numpy()
-        out[prec] = dict(normwise=float(torch.sqrt((num ** 2).sum() / (den ** 2).sum())),
-                         p99=float(np.quantile(per, 0.99)), max=float(per.max()))
+        out[name] = dict(normwise=float(torch.sqrt((num ** 2).sum() / (den ** 2).sum())),
+                         p99=float(np.quantile(per, 0.99)), max=float(per.max()),
+                         n_over_1e5=int((per > 1e-5).sum()))
+        if kap > 0:
+            out[name]["recomputed"] = int(eng.ctx.stats()[6])
     eng.check()
     # FP64 GPU vs the CPU oracle on sampled particles
     from oracle import oracle as orc
@@ -215,6 +238,7 @@ def batched_line(steps, model_path, seed=0):
                            tuning="auto", autotune_candidates=BATCHED_CANDIDATES)
         out["tuning"] = b.tune_result["tuning"]
         sec = b.run(steps, warmup=1)
+        out["recomputed_last_step"] = int(b.eng.ctx.stats()[6])
         inter = b.eng.interactions()[-1].astype(np.float64)    # placement of the last step
         out[name] = {"ms_per_step": sec * 1e3, "value": float(sizes.sum()) / sec / 1e6,
                      "interactions_last_step": float(inter.sum()),
@@ -282,7 +306,11 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--recompute-kappa", type=float, default=RECOMPUTE_KAPPA,
+                    help="FP32 paths: FP64 recompute of particles with error bound > kappa*|a| "
+                         "(0 = off)")
     args = ap.parse_args()
+    kappa = args.recompute_kappa
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -324,12 +352,15 @@ def main():
             def order_fn():
                 eng.ctx.learned_order(order_model.params, order_buf)
                 return order_buf
-        sec, kms, near, far = run_path(eng, precision, args.steps, args.warmup, order_fn)
-        n_items = int(eng.ctx.stats()[2])
+        sec, kms, rms, near, far = run_path(eng, precision, args.steps, args.warmup, order_fn,
+                                            kappa)
+        st = eng.ctx.stats()
+        n_items, n_rc = int(st[2]), int(st[6])
         eng.close()
         flops = FLOP_PER_INTERACTION * (near + far).astype(np.float64)
         return dict(seconds_per_step=sec, value=args.n / sec / 1e6, kms=kms,
-                    force_kernel_ms=float(kms.mean()), near=near, far=far,
+                    force_kernel_ms=float(kms.mean()), recompute_ms=float(rms.mean()),
+                    recomputed_last_step=n_rc, near=near, far=far,
                     flop_per_launch=float(flops.mean()),
                     achieved=float(flops.sum() / (kms.sum() * 1e-3) / 1e12), tuning=tun,
                     work_items=n_items)
@@ -379,9 +410,13 @@ def main():
                    "mass": "U[1,2]", "G": GC, "dt": DT, "eps2": EPS2, "r2_cut": R2CUT,
                    "seed": args.seed, "schedule": sched, "cost_model":
                        os.path.relpath(args.cost_model, ROOT) if model is not None else None,
-                   "tuning": res["tuning"], "work_items": res["work_items"]},
+                   "tuning": res["tuning"], "work_items": res["work_items"],
+                   "recompute_kappa": kappa},
         "interactions_per_step": {"near": float(res["near"].mean()),
                                   "far": float(res["far"].mean())},
+        "recompute": {"kappa": kappa, "ms_per_step": res["recompute_ms"],
+                      "share_of_step": res["recompute_ms"] / (res["seconds_per_step"] * 1e3),
+                      "particles_last_step": res["recomputed_last_step"]},
         "roofline": {"bound": "valu", "achieved": res["achieved"], "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": res["achieved"] / FP32_PEAK_TFLOPS,
                      "traffic": TRAFFIC, "kernel": "k_force32<P,0,1>",
@@ -394,6 +429,7 @@ def main():
         out["fp32_static"] = {"value": res_static["value"],
                               "ms_per_step": res_static["seconds_per_step"] * 1e3,
                               "force_kernel_ms": res_static["force_kernel_ms"],
+                              "recompute_ms": res_static["recompute_ms"],
                               "achieved_tflops": res_static["achieved"]}
         out["learned_vs_static"] = res_static["seconds_per_step"] / res["seconds_per_step"]
     if res64 is not None:
@@ -404,7 +440,7 @@ def main():
         out["speedup_vs_fp64"] = res64["seconds_per_step"] / res["seconds_per_step"]
     if not args.skip_errors:
         log("force errors")
-        out["force_rel_err"] = force_errors(d, G, Engine)
+        out["force_rel_err"] = force_errors(d, G, Engine, kappa=kappa)
     if not args.skip_extra:
         log("batched systems")
         out["batched_512"] = batched_line(args.steps, args.batched_model)

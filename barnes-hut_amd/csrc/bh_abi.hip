@@ -89,8 +89,7 @@ struct bh_ctx_s {
     double rc_kappa = 2.0;
     int rc_deferred = 0;
     int rc_ready = 0;        // the last force pass prepared a32 / s2n
-    double* rc_a32 = nullptr;
-    double* rc_s2n = nullptr;
+    double* rc_a32 = nullptr;      // the FP32 pass: x plane, y plane, near-field sums
     int32_t* rc_list = nullptr;    // flagged particles, system s at [p_off_s, p_off_s + cnt_s)
     int32_t* rc_cnt = nullptr;     // per system
     int32_t* rc_gofs = nullptr;    // per system: first group (S + 1)
@@ -255,8 +254,7 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     A(cost_scratch, 16 + 3 * 4096 + 1);
     A(feat, BH_MLP_NFEAT * c->max_items);
     A(pred, c->max_items);
-    A(rc_a32, 2 * N);
-    A(rc_s2n, N);
+    A(rc_a32, 3 * N);
     A(rc_list, N);
     A(rc_cnt, nsys);
     A(rc_gofs, nsys + 1);
@@ -578,8 +576,13 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     fa.part = c->part;
     fa.part_n = c->N;
     const bool rc = fp32v && c->rc_kappa > 0.0;
-    fa.a32 = rc ? c->rc_a32 : nullptr;
-    fa.s2n = rc ? c->rc_s2n : nullptr;
+    fa.rc = rc ? 1 : 0;
+    if (rc) {   // the FP32 result goes to scratch; the recompute adds it (or the FP64 one)
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(2 * c->N, 256, 8192)), dim3(256), 0, st,
+                           c->rc_a32, 2 * c->N, 0.0);
+        fa.ax = c->rc_a32;
+        fa.ay = c->rc_a32 + c->N;
+    }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
     void* args[] = {&fa};
     unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
@@ -589,8 +592,7 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
         hipLaunchKernelGGL(k_force_combine, dim3(grid_for((int64_t)fa.n_tail * pslots, 256,
                                                           (int64_t)c->num_cu * 32)), dim3(256),
                            0, st, (const Item*)c->items, order, (const int32_t*)c->stats, fa.split,
-                           fa.n_tail, pslots, (const double*)c->part, fa.part_n, p->ax, p->ay,
-                           fa.a32);
+                           fa.n_tail, pslots, (const double*)c->part, fa.part_n, fa.ax, fa.ay);
     }
     if (!rc)   // the recomputed count describes the last pass
         hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_NRECOMP,
@@ -624,7 +626,7 @@ int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stre
     if (!c->rc_ready) return fail(BH_ESTATE, "no FP32 force pass prepared a recompute");
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_NRECOMP, (int64_t)2, 0);
-    if (law->G == 0.0) return BH_OK;   // every term is zero
+    if (law->G == 0.0) return BH_OK;   // every term, and so the FP32 result, is zero
     hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c->S, 64, 1024)), dim3(64), 0, st, c->rc_cnt,
                        (int64_t)c->S, 0);
     hipLaunchKernelGGL(k_rc_blocks, dim3(grid_for(c->nblocks, 1, (int64_t)c->num_cu * 16)),
@@ -643,9 +645,9 @@ int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stre
                        (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
                        (const double*)c->com_dense, (const double*)c->rc_tiles,
                        (const double*)c->rc_blk, (const double*)c->rc_bfar,
-                       (const double*)c->rc_a32,
-                       (const double*)c->rc_s2n, law->G, c->rc_kappa * c->rc_kappa,
-                       (const int32_t*)c->stats, c->rc_cnt, c->rc_list);
+                       (const double*)c->rc_a32, (const double*)(c->rc_a32 + c->N),
+                       (const double*)(c->rc_a32 + 2 * c->N), law->G, c->rc_kappa * c->rc_kappa,
+                       (const int32_t*)c->stats, c->rc_cnt, c->rc_list, p->ax, p->ay);
     hipLaunchKernelGGL(k_rc_groups, dim3(1), dim3(256), 0, st, (const int32_t*)c->rc_cnt, c->S,
                        c->rc_gofs, c->stats);
     const unsigned g = (unsigned)((int64_t)c->num_cu * 4);
@@ -656,8 +658,8 @@ int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stre
                            (const SysDev*)c->sys, (const int32_t*)c->key_sorted,
                            (const int32_t*)c->offsets, (const int32_t*)c->neprefix,
                            (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
-                           (const double*)p->mass, (const double*)c->rc_a32, law->G, law->eps2,
-                           law->r2_cut, c->rc_part, p->ax, p->ay);
+                           (const double*)p->mass, law->G, law->eps2, law->r2_cut, c->rc_part,
+                           p->ax, p->ay);
     else
         hipLaunchKernelGGL(k_rc_fix64<0>, dim3(g), dim3(RC_THREADS), 0, st,
                            (const int32_t*)c->stats, (const int32_t*)c->rc_cnt,
@@ -665,13 +667,12 @@ int bh_recompute(void* ctx, const bh_particles* p, const bh_law* law, void* stre
                            (const SysDev*)c->sys, (const int32_t*)c->key_sorted,
                            (const int32_t*)c->offsets, (const int32_t*)c->neprefix,
                            (const Com64*)c->l64, (const double*)p->px, (const double*)p->py,
-                           (const double*)p->mass, (const double*)c->rc_a32, law->G, law->eps2,
-                           law->r2_cut, c->rc_part, p->ax, p->ay);
+                           (const double*)p->mass, law->G, law->eps2, law->r2_cut, c->rc_part,
+                           p->ax, p->ay);
     hipLaunchKernelGGL(k_rc_combine, dim3(grid_for((int64_t)RC_PART_CAP * RC_GROUP, 256, 512)),
                        dim3(256), 0, st, (const int32_t*)c->stats, (const int32_t*)c->rc_cnt,
                        (const int32_t*)c->rc_gofs, c->S, (const int32_t*)c->rc_list,
-                       (const SysDev*)c->sys, (const double*)c->rc_part, (const double*)c->rc_a32,
-                       law->G, p->ax, p->ay);
+                       (const SysDev*)c->sys, (const double*)c->rc_part, law->G, p->ax, p->ay);
     LAUNCH_CHECK("recompute");
     return BH_OK;
 }

@@ -469,10 +469,10 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
             }
         }
 
-        if (A.s2n && part <= 0) {
+        if (A.rc && part <= 0) {
 #pragma unroll
             for (int k = 0; k < P; ++k)
-                if (act[k]) A.s2n[gi[k]] = (double)s2n[k] * ((double)msc * (double)msc);
+                if (act[k]) A.ay[A.part_n + gi[k]] = (double)s2n[k] * ((double)msc * (double)msc);
         }
 
         // far field (a split item's part covers one 64-aligned slice of the list)
@@ -624,10 +624,6 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                 if (part < 0) {
                     A.ax[gi[k]] += scale * rx;
                     A.ay[gi[k]] += scale * ry;
-                    if (A.a32) {
-                        A.a32[2 * (int64_t)gi[k]] = scale * rx;
-                        A.a32[2 * (int64_t)gi[k] + 1] = scale * ry;
-                    }
                 } else {
                     const int64_t o = ((int64_t)part * A.part_n + gi[k]) * 2;
                     A.part[o] = scale * rx;
@@ -647,8 +643,7 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
                                                        const double* __restrict__ part,
                                                        int64_t part_n,
                                                        double* __restrict__ ax,
-                                                       double* __restrict__ ay,
-                                                       double* __restrict__ a32)
+                                                       double* __restrict__ ay)
 {
     const int nitems = stats[ST_NITEMS];
     const int nt = min(n_tail, nitems);
@@ -671,10 +666,6 @@ __global__ __launch_bounds__(256) void k_force_combine(const Item* __restrict__ 
         }
         ax[gi] += sx;
         ay[gi] += sy;
-        if (a32) {
-            a32[2 * gi] = sx;
-            a32[2 * gi + 1] = sy;
-        }
     }
 }
 

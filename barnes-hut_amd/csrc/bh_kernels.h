@@ -35,12 +35,13 @@ struct ForceArgs {
     int split, n_tail;
     double* part;
     int64_t part_n;
-    // FP64 recompute inputs (FP32 kernels; NULL when the recompute is off):
-    // a32 receives each particle's own FP32 result (2 doubles per particle),
-    // s2n the sum over its near-field terms of (m_j / r_ij)^2 (an upper bound
-    // of sum |term|^2 under any softening or cutoff).
-    double* a32;
-    double* s2n;
+    // FP64 recompute on (FP32 kernels): ax / ay point at zeroed scratch
+    // planes of part_n doubles, and ay[part_n + i] receives particle i's sum
+    // over its near-field terms of (m_j / r_ij)^2 (an upper bound of sum
+    // |term|^2 under any softening or cutoff); the recompute then adds the
+    // FP32 or the FP64 result to the particles' accelerations.  A flag rather
+    // than a third pointer keeps the kernel's scalar registers unchanged.
+    int rc;
 };
 
 // One FP64 weight m * w(r2) of the law (LAW 0: no cutoff; LAW 1: r2 <= r2cut
@@ -112,7 +113,7 @@ __global__ void k_energy(const double* x, const double* y, const double* m, cons
 __global__ void k_clamp_items(int32_t* stats, int64_t max_items);
 __global__ void k_force_combine(const Item* items, const int32_t* order, const int32_t* stats,
                                 int split, int n_tail, int pslots, const double* part,
-                                int64_t part_n, double* ax, double* ay, double* a32);
+                                int64_t part_n, double* ax, double* ay);
 // FP64 recompute of the particles whose FP32 rounding-error estimate is
 // large against their acceleration
 __global__ void k_rc_blocks(const SysDev* sys, int S, int64_t nblocks, const double* com_dense,
@@ -124,21 +125,21 @@ __global__ void k_rc_blockfar(const SysDev* sys, int S, int64_t nblocks, const d
 __global__ void k_rc_flag(const Item* items, const SysDev* sys, const int32_t* key_sorted,
                           const int32_t* neprefix, const Com64* l64, const double* x,
                           const double* y, const double* com, const double* tl,
-                          const double* blk, const double* bfar,
-                          const double* a32, const double* s2n,
-                          double Gc, double kappa2, const int32_t* stats, int32_t* rc_cnt,
-                          int32_t* list);
+                          const double* blk, const double* bfar, const double* a32x,
+                          const double* a32y, const double* s2n, double Gc, double kappa2,
+                          const int32_t* stats, int32_t* rc_cnt, int32_t* list, double* ax,
+                          double* ay);
 __global__ void k_rc_groups(const int32_t* rc_cnt, int S, int32_t* gofs, int32_t* stats);
 template <int LAW>
 __global__ void k_rc_fix64(const int32_t* stats, const int32_t* rc_cnt, const int32_t* gofs,
                            int S, const int32_t* list, const SysDev* sys,
                            const int32_t* key_sorted, const int32_t* offsets,
                            const int32_t* neprefix, const Com64* l64, const double* x,
-                           const double* y, const double* m, const double* a32, double Gc,
-                           double eps2, double r2cut, double* part, double* ax, double* ay);
+                           const double* y, const double* m, double Gc, double eps2,
+                           double r2cut, double* part, double* ax, double* ay);
 __global__ void k_rc_combine(const int32_t* stats, const int32_t* rc_cnt, const int32_t* gofs,
                              int S, const int32_t* list, const SysDev* sys, const double* part,
-                             const double* a32, double Gc, double* ax, double* ay);
+                             double Gc, double* ax, double* ay);
 
 template <bool WRITE>
 __global__ void k_items(const SysDev* sys, int S, int64_t ncols, const int32_t* counts,

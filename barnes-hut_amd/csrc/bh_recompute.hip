@@ -9,8 +9,10 @@
 // stays ~1e-7.  After an FP32 pass, k_rc_flag bounds sqrt(sum_j |term_j|^2)
 // from above for every particle and flags those whose bound exceeds kappa
 // times their acceleration; k_rc_fix64 evaluates the flagged particles in
-// FP64 (the law's terms with a Newton-refined reciprocal) and replaces their
-// FP32 result.
+// FP64 (the law's terms with a Newton-refined reciprocal).  The FP32 kernels
+// write into scratch planes (a32x, a32y); each particle's acceleration gets
+// either its FP32 result (k_rc_flag) or its FP64 one (k_rc_fix64 /
+// k_rc_combine) added.
 //
 // The bound, for a work item (particles of column cx, rows [y0, y1]; box
 // [cx, cx + 1] x [y0, y1 + 1] in cell units):
@@ -157,10 +159,12 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
                                                 const double* __restrict__ tl,
                                                 const double* __restrict__ blk,
                                                 const double* __restrict__ bfar,
-                                                const double* __restrict__ a32,
+                                                const double* __restrict__ a32x,
+                                                const double* __restrict__ a32y,
                                                 const double* __restrict__ s2n, double Gc,
                                                 double kappa2, const int32_t* __restrict__ stats,
-                                                int32_t* rc_cnt, int32_t* __restrict__ list)
+                                                int32_t* rc_cnt, int32_t* __restrict__ list,
+                                                double* __restrict__ ax, double* __restrict__ ay)
 {
     __shared__ Com64 tile[BH_WAVE];
     const int lane = threadIdx.x;
@@ -290,12 +294,16 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
             const double xi = x[gi], yi = y[gi];
             const int cy = (int)(key_sorted[gi] - coff) - cx * G;
             const double sn = s2n[gi];
-            const double axv = a32[2 * gi], ayv = a32[2 * gi + 1];
+            const double axv = a32x[gi], ayv = a32y[gi];
             const double A2 = fma(axv, axv, ayv * ayv) * g2;
             // first test with the item-level bound of the exact region; the
             // exact per-particle sum only runs where some lane passes it
             const double loose = psi + preg + sn;
             const bool cand = act && loose != 0.0 && !(loose <= kappa2 * A2);
+            if (act && !cand) {
+                ax[gi] += axv;
+                ay[gi] += ayv;
+            }
             if (__ballot(cand) == 0ull) continue;
             double est = psi + sn;
             // far cells of the exact region, per particle (LDS tiles)
@@ -320,6 +328,10 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
             }
             // NaN or infinite estimates flag too: the FP64 terms decide
             const bool flag = cand && est != 0.0 && !(est <= kappa2 * A2);
+            if (cand && !flag) {
+                ax[gi] += axv;
+                ay[gi] += ayv;
+            }
             const unsigned long long mask = __ballot(flag);
             const int n = __popcll(mask);
             if (n == 0) continue;
@@ -400,8 +412,8 @@ __device__ __forceinline__ int rc_sys_of_group(const int32_t* gofs, int S, int g
 
 // One block per (group, slice) (grid-stride).  The threads split the slice of
 // the system's COM list and, in slice 0, the near cells; the block's sums are
-// reduced in a fixed order.  With one slice the FP32 result (a32) is swapped
-// for the FP64 one here, else k_rc_combine does it.
+// reduced in a fixed order.  With one slice the FP64 result is added to the
+// acceleration here, else k_rc_combine adds it.
 template <int LAW>
 __global__ __launch_bounds__(RC_THREADS) void k_rc_fix64(
     const int32_t* __restrict__ stats, const int32_t* __restrict__ rc_cnt,
@@ -409,8 +421,8 @@ __global__ __launch_bounds__(RC_THREADS) void k_rc_fix64(
     const SysDev* __restrict__ sys, const int32_t* __restrict__ key_sorted,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ neprefix,
     const Com64* __restrict__ l64, const double* __restrict__ x, const double* __restrict__ y,
-    const double* __restrict__ m, const double* __restrict__ a32, double Gc, double eps2,
-    double r2cut, double* __restrict__ part, double* ax, double* ay)
+    const double* __restrict__ m, double Gc, double eps2, double r2cut,
+    double* __restrict__ part, double* ax, double* ay)
 {
     constexpr int NW = RC_THREADS / BH_WAVE;
     __shared__ double red[NW][2 * RC_GROUP];
@@ -500,7 +512,7 @@ __global__ __launch_bounds__(RC_THREADS) void k_rc_fix64(
                 if (k < cnt) {
                     const int64_t gi = list[lb + k];
                     double* a = c ? ay : ax;
-                    a[gi] = (a[gi] - a32[2 * gi + c]) + Gc * r;
+                    a[gi] += Gc * r;
                 }
             } else {
                 part[w * (2 * RC_GROUP) + tid] = r;
@@ -510,15 +522,14 @@ __global__ __launch_bounds__(RC_THREADS) void k_rc_fix64(
     }
 }
 
-// Adds the slices' partial sums in slice order and swaps the FP32 result of
-// each flagged particle for the FP64 one (only when slices were used).
+// Adds the slices' partial sums in slice order and the FP64 result to each
+// flagged particle's acceleration (only when slices were used).
 __global__ __launch_bounds__(256) void k_rc_combine(const int32_t* __restrict__ stats,
                                                     const int32_t* __restrict__ rc_cnt,
                                                     const int32_t* __restrict__ gofs, int S,
                                                     const int32_t* __restrict__ list,
                                                     const SysDev* __restrict__ sys,
-                                                    const double* __restrict__ part,
-                                                    const double* __restrict__ a32, double Gc,
+                                                    const double* __restrict__ part, double Gc,
                                                     double* ax, double* ay)
 {
     const int ngroups = stats[ST_RCGROUPS];
@@ -538,8 +549,8 @@ __global__ __launch_bounds__(256) void k_rc_combine(const int32_t* __restrict__ 
             rx += p[2 * k];
             ry += p[2 * k + 1];
         }
-        ax[gi] = (ax[gi] - a32[2 * gi]) + Gc * rx;
-        ay[gi] = (ay[gi] - a32[2 * gi + 1]) + Gc * ry;
+        ax[gi] += Gc * rx;
+        ay[gi] += Gc * ry;
     }
 }
 
@@ -548,7 +559,6 @@ __global__ __launch_bounds__(256) void k_rc_combine(const int32_t* __restrict__ 
                                            const int32_t*, const SysDev*, const int32_t*,      \
                                            const int32_t*, const int32_t*, const Com64*,       \
                                            const double*, const double*, const double*,        \
-                                           const double*, double, double, double, double*,     \
-                                           double*, double*);
+                                           double, double, double, double*, double*, double*);
 RC_FIX64_INST(0)
 RC_FIX64_INST(1)

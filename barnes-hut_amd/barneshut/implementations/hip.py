@@ -25,6 +25,8 @@ Extra keys, section [hip]:
   cost_model  .npz file of a trained cost model (schedule = learned)
   p, yspan, waves_per_cu, tail_split, tail_items   force-kernel launch shape
               (autotune = True to tune)
+  recompute_kappa  FP32 precisions: FP64 recompute of the particles whose
+              rounding-error bound exceeds kappa * |a| (default 2; 0 = off)
 """
 import numpy as np
 
@@ -50,6 +52,7 @@ class HipBarnesHut(BaseBarnesHut):
                            tail_split=int(Config.get("hip", "tail_split", fallback="16")),
                            tail_items=int(Config.get("hip", "tail_items", fallback="1048576")))
         self.device = int(Config.get("hip", "device", fallback="0"))
+        self.recompute_kappa = float(Config.get("hip", "recompute_kappa", fallback="2.0"))
         self.engine = None
         self.model = None
         self.order = None
@@ -63,6 +66,7 @@ class HipBarnesHut(BaseBarnesHut):
         self.grid_dim_ = self.grid_dim()
         self.engine = Engine([self.n_particles], [self.grid_dim_], device=self.device,
                              tuning=self.tuning)
+        self.engine.ctx.set_recompute(self.recompute_kappa)
         a = self.particles
         self.engine.load(dict(px=a[:, p.px], py=a[:, p.py], mass=a[:, p.mass], vx=a[:, p.vx],
                               vy=a[:, p.vy], ax=a[:, p.ax], ay=a[:, p.ay],

@@ -55,7 +55,7 @@ FLOP_PER_INTERACTION = 11
 # 1 GiB device copy (FETCH x2.0, WRITE x1.0), measured on the first two
 # steps of this workload with the default launch shape
 # (profiles/force32_gauss64_n4M_g1024_hbm_traffic.md).
-TRAFFIC = 3.47e9
+TRAFFIC = 5.10e9
 # Issue-limited ceiling of the FP32 far-field term: 6 VALU instructions per
 # interaction (v_rcp_f32 co-issued), no packed FP32, 256 CUs x 4 SIMDs x 16
 # lanes x 2.4 GHz = 39.3e12 lane-instructions/s -> 6.55e12 interactions/s.

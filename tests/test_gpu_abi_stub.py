@@ -1,7 +1,8 @@
 """The raw ctypes binding a maintainer would add on the reference side (the
 stub shown for the C ABI), exercised as written: no helper module, plain
 pointers and a stream handle.  One FP64 round through the per-kernel entry
-points and one through the captured whole-step graph, against the oracle."""
+points and one through the captured whole-step graph, against the oracle, and
+an FP32 pass with the FP64 recompute run inside the pass or by the caller."""
 import ctypes
 import os
 
@@ -80,6 +81,39 @@ def test_stub_binding_phases_and_graph():
     np.testing.assert_array_equal(h["pid"], ref["pid"])
     err = np.hypot(h["ax"] - ref["ax"], h["ay"] - ref["ay"]) / np.hypot(ref["ax"], ref["ay"])
     assert err.max() < 1e-9
+    # FP32 pass with the FP64 recompute run by the caller (deferred) equals the
+    # same pass with the recompute inside bh_force_pass
+    lib.bh_set_recompute.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int32]
+    lib.bh_set_recompute.restype = ctypes.c_int
+    lib.bh_recompute.argtypes = [ctypes.c_void_p, ctypes.POINTER(Particles), ctypes.POINTER(Law),
+                                 ctypes.c_void_p]
+    lib.bh_recompute.restype = ctypes.c_int
+    lib.bh_read_stats.restype = ctypes.c_int
+    fp32 = []
+    for deferred in (0, 1):
+        tb["ax"].zero_()
+        tb["ay"].zero_()
+        assert lib.bh_set_recompute(ctx, ctypes.c_double(2.0), deferred) == 0
+        assert lib.bh_force_pass(ctx, ctypes.byref(b), ctypes.byref(law), 1, None, stream) == 0
+        if deferred:
+            assert lib.bh_recompute(ctx, ctypes.byref(b), ctypes.byref(law), stream) == 0
+        stats = (ctypes.c_int32 * 16)()
+        assert lib.bh_read_stats(ctx, stats, stream) == 0
+        fp32.append((tb["ax"].cpu().numpy(), tb["ay"].cpu().numpy(), stats[6]))
+    np.testing.assert_array_equal(fp32[0][0], fp32[1][0])
+    np.testing.assert_array_equal(fp32[0][1], fp32[1][1])
+    assert fp32[0][2] == fp32[1][2]
+    err = np.hypot(fp32[0][0] - ref["ax"], fp32[0][1] - ref["ay"]) / np.hypot(ref["ax"], ref["ay"])
+    assert err.max() < 1e-5
+    # without a prepared FP32 pass the recompute is refused (BH_ESTATE)
+    assert lib.bh_set_recompute(ctx, ctypes.c_double(2.0), 1) == 0
+    assert lib.bh_recompute(ctx, ctypes.byref(b), ctypes.byref(law), stream) == 4
+    assert lib.bh_set_recompute(ctx, ctypes.c_double(2.0), 0) == 0
+
+    # back to the FP64 accelerations for the tick below
+    tb["ax"].zero_()
+    tb["ay"].zero_()
+    assert lib.bh_force_pass(ctx, ctypes.byref(b), ctypes.byref(law), 0, None, stream) == 0
     assert lib.bh_tick_particles(ctx, ctypes.byref(b), ctypes.c_double(1e-4), stream) == 0
     torch.cuda.synchronize()
     after_phases = {k: tb[k].cpu().numpy() for k in ("px", "py", "vx", "vy", "pid")}

@@ -299,7 +299,10 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
             // first test with the item-level bound of the exact region; the
             // exact per-particle sum only runs where some lane passes it
             const double loose = psi + preg + sn;
-            const bool cand = act && loose != 0.0 && !(loose <= kappa2 * A2);
+            // a non-finite FP32 result (overflow, coincident particles) always
+            // takes the FP64 path
+            const bool bad = !(A2 <= 1.7976931348623157e308);
+            const bool cand = act && ((loose != 0.0 && !(loose <= kappa2 * A2)) || bad);
             if (act && !cand) {
                 ax[gi] += axv;
                 ay[gi] += ayv;
@@ -327,7 +330,7 @@ __global__ __launch_bounds__(64) void k_rc_flag(const Item* __restrict__ items,
                 __syncthreads();
             }
             // NaN or infinite estimates flag too: the FP64 terms decide
-            const bool flag = cand && est != 0.0 && !(est <= kappa2 * A2);
+            const bool flag = cand && ((est != 0.0 && !(est <= kappa2 * A2)) || bad);
             if (cand && !flag) {
                 ax[gi] += axv;
                 ay[gi] += ayv;

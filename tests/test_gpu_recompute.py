@@ -221,3 +221,31 @@ def test_recompute_off_leaves_fp32_pass_unchanged():
     same = (on[0] == off[0]) & (on[1] == off[1])
     assert same.sum() >= n - 1000
     eng.close()
+
+
+def test_fp32_overflow_takes_fp64_path():
+    """A light particle 1e-9 from a 1e36-mass neighbour: its FP32 near term
+    overflows to inf (mass ratio and separation outside FP32's range) while
+    the FP64 value is finite; the recompute gives it the FP64 value."""
+    rng = np.random.default_rng(308)
+    n = 200
+    d = {"px": rng.random(n), "py": rng.random(n), "mass": rng.uniform(1.0, 2.0, n),
+         "vx": np.zeros(n), "vy": np.zeros(n), "pid": np.arange(n, dtype=np.int32)}
+    # heavy particle in its own cell next to the light one (cell boundary between)
+    d["px"][0], d["py"][0], d["mass"][0] = 0.5 - 0.5e-9, 0.5, 1.0
+    d["px"][1], d["py"][1], d["mass"][1] = 0.5 + 0.5e-9, 0.5, 1e36
+    d["px"][2], d["py"][2] = 0.0, 0.0           # fix the box to [0, 1]
+    d["px"][3], d["py"][3] = 1.0, 1.0
+    law = (1.0, 0.0, 0.0)
+    eng = _placed(d, [n], [8])
+    r64 = _pass(eng, "fp64", law)
+    r32 = _pass(eng, "fp32", law, kappa=0.0)
+    rrc = _pass(eng, "fp32", law, kappa=2.0)
+    pid = eng.fetch()["pid"]
+    eng.close()
+    i = int(np.nonzero(pid == 0)[0][0])
+    print(f"[overflow] fp64 {r64[0][i]:.3e}, fp32 {r32[0][i]:.3e}, recompute {rrc[0][i]:.3e}",
+          flush=True)
+    assert np.isfinite(r64[0][i]) and np.isfinite(r64[1][i])
+    assert not np.isfinite(r32[0][i])
+    assert _rel(rrc[0][i:i + 1], rrc[1][i:i + 1], r64[0][i:i + 1], r64[1][i:i + 1])[0] < 1e-9

@@ -24,9 +24,12 @@ BATCHED_TUNING = dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=
 class BatchedSystems:
 
     def __init__(self, sizes, grid_dims, data, law=(1.0, 0.0, 0.0), precision="fp32", dt=0.0,
-                 tuning=None, device=0, model=None, graph=True, autotune_candidates=None):
+                 tuning=None, device=0, model=None, graph=True, autotune_candidates=None,
+                 recompute_kappa=2.0):
         """tuning: None (BATCHED_TUNING), a dict, or "auto": the autotuner picks
-        the launch shape on the loaded state (autotune_candidates narrows it)."""
+        the launch shape on the loaded state (autotune_candidates narrows it).
+        recompute_kappa: FP32 precisions re-evaluate in FP64 the particles whose
+        rounding-error bound exceeds kappa * |a| (0 turns it off)."""
         self.sizes = np.asarray(sizes, np.int64)
         self.grid_dims = np.asarray(grid_dims, np.int32)
         auto = isinstance(tuning, str)
@@ -34,6 +37,7 @@ class BatchedSystems:
             raise ValueError(f"tuning must be None, a dict or 'auto', not {tuning!r}")
         self.eng = Engine(self.sizes, self.grid_dims, device=device,
                           tuning=BATCHED_TUNING if (tuning is None or auto) else tuning)
+        self.eng.ctx.set_recompute(recompute_kappa)
         self.eng.load(data)
         self.tune_result = None
         if auto:

@@ -21,11 +21,19 @@ CACHE = os.path.join(os.path.expanduser("~"), ".cache", "bh_mi355x_autotune.json
 
 
 def _key(eng, precision):
+    """Device, systems, precision and the library build (a rebuilt kernel is
+    tuned again)."""
     import torch
+    from .kernels import hiplib
     name = torch.cuda.get_device_name(eng.dev)
     n = ",".join(str(int(v)) for v in eng.n[:8])
     g = ",".join(str(int(v)) for v in eng.grid_dim[:8])
-    return f"{name}|S={len(eng.n)}|n={n}|G={g}|{precision}"
+    try:
+        st = os.stat(hiplib.LIB_PATH)
+        build = f"{st.st_size}:{int(st.st_mtime)}"
+    except OSError:
+        build = "?"
+    return f"{name}|S={len(eng.n)}|n={n}|G={g}|{precision}|lib={build}"
 
 
 def _time_force(eng, law, precision, order=None, reps=2):

@@ -242,3 +242,51 @@ def test_4m_workloads_full_size_sampled_against_oracle(name, nc, sigma, mass, se
           f"{np.quantile(num / np.hypot(rx, ry), 0.99):.3e}", flush=True)
     assert norm <= 1e-5
     eng.close()
+
+
+def test_bench_workload_fp32_trajectory_diverges_like_a_1e7_law_perturbation():
+    """The bench workload at full size (4,194,304 particles, 64 clusters,
+    1024 x 1024 grid, G = 1e-6, dt = 1e-3, no softening), 10 full steps on the
+    FP32 path (with the FP64 recompute), on the FP64 path, and on the FP64 path
+    with G multiplied by 1 + 1e-7 (an exact, uniform 1e-7 force perturbation).
+    After one step the FP32 velocities match FP64 to the force error (~1e-7).
+    Later, unresolved close pairs (dt far above their orbital time, no
+    softening) amplify any difference and move particles across cell
+    boundaries, which changes COMs for everyone: the FP64 path perturbed by
+    1e-7 diverges from itself by ~1e-2 normwise after 10 steps (measured on
+    MI355X: 9.5e-3, median 1.6e-5; a 1e-10 perturbation gives 5.7e-4).  The
+    FP32 path may not diverge more than that perturbed FP64 path does."""
+    import torch
+    n, G = 4194304, 1024
+    d = inputs.gaussian_mixture(n, 64, (0.005, 0.05), seed=2024)
+    runs = {"fp64": ("fp64", GC), "fp64_law": ("fp64", GC * (1 + 1e-7)), "fp32": ("fp32", GC)}
+    vel = {}
+    for name, (prec, gc) in runs.items():
+        eng = _engine(n, G)
+        eng.load(d)
+        vel[name] = []
+        for s in range(10):
+            eng.step(gc, 0.0, 0.0, prec, DT, graph=True)
+            if s in (0, 9):
+                h = eng.fetch()
+                o = np.argsort(h["pid"])
+                vel[name].append((h["vx"][o], h["vy"][o]))
+        eng.check()
+        eng.close()
+
+    def norm(a, b):
+        return float(np.sqrt(((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2).sum()
+                             / (b[0] ** 2 + b[1] ** 2).sum()))
+
+    def med(a, b):
+        return float(np.median(np.hypot(a[0] - b[0], a[1] - b[1]) / np.hypot(b[0], b[1])))
+
+    one = norm(vel["fp32"][0], vel["fp64"][0])
+    d32, dlaw = norm(vel["fp32"][1], vel["fp64"][1]), norm(vel["fp64_law"][1], vel["fp64"][1])
+    m32, mlaw = med(vel["fp32"][1], vel["fp64"][1]), med(vel["fp64_law"][1], vel["fp64"][1])
+    print(f"[trajectory] step 1: fp32 vs fp64 {one:.2e}; step 10: fp32 vs fp64 {d32:.2e} "
+          f"(median {m32:.1e}), fp64 with G(1+1e-7) vs fp64 {dlaw:.2e} (median {mlaw:.1e})",
+          flush=True)
+    assert one <= 1e-6
+    assert d32 <= 2.0 * dlaw
+    assert m32 <= 2.0 * mlaw

@@ -60,9 +60,11 @@ struct Com64 {
 };
 
 // FP32 variant: COM expressed in cell units relative to its own cell corner
-// (X = (kx + fx) * step + min_x).
+// (X = (kx + fx) * step + min_x), and the reciprocal of the cell's mass
+// rM = 1 / (float)M (IEEE division, once per cell and step; +inf for M = 0):
+// the FP32 kernels need s / M for a power of two s, which is s * rM exactly.
 struct Com32 {
-    float fx, fy, M;
+    float fx, fy, rM;
     int32_t kxy;   // (kx << 16) | ky
 };
 

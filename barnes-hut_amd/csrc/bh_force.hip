@@ -147,12 +147,14 @@ __device__ __forceinline__ bool meets(int b0, int b1, int lo, int hi)
     return lo < hi && b0 < hi && b1 > lo;
 }
 
+// M w(s2) of a far entry from its reciprocal mass rM = 1 / M (+inf for the
+// padding and zero-mass cells, whose terms are then exactly 0).
 template <int LAW>
-__device__ __forceinline__ float far_t32(float wx, float wy, float M, float eps2c, float r2c)
+__device__ __forceinline__ float far_t32(float wx, float wy, float rM, float eps2c, float r2c)
 {
-    if (LAW == 0) return M * __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, eps2c)));
+    if (LAW == 0) return __builtin_amdgcn_rcpf(fmaf(wx, wx, fmaf(wy, wy, eps2c)) * rM);
     const float s2 = fmaf(wx, wx, wy * wy);
-    return (s2 > r2c) ? M * __builtin_amdgcn_rcpf(s2 + eps2c) : 0.0f;
+    return (s2 > r2c) ? __builtin_amdgcn_rcpf((s2 + eps2c) * rM) : 0.0f;
 }
 
 // ---------------------------------------------------------------- FP64
@@ -325,8 +327,11 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
 // for t, two for |t|^2, v_rcp_f32 and two fma into the sums: one VALU
 // instruction fewer than forming M * w.  Scaling by the power of two s is
 // exact, so s only keeps c^2 |a - u|^2 inside the FP32 range (s = 2^e with
-// 2^e <= M of the item's own cell < 2^(e+1)); near-field terms are divided by s
-// (exactly) to share the sums, and the sums are multiplied by s at the end.
+// M <= 2^e < 2 M for the item's own cell; any power of two gives the same
+// sums bit for bit); near-field terms are divided by s (exactly) to share the
+// sums, and the sums are multiplied by s at the end.  c = s * rM from the
+// list's reciprocal mass equals the correctly rounded s / M (a power-of-two
+// scaling commutes with rounding), so staging a tile takes no division.
 // Zero-mass entries and the padding past the list hold (1e30, 1e30, 0, 0):
 // |t|^2 overflows to inf, v_rcp_f32 gives 0 and the term adds exactly 0.
 // Minimum resident waves per SIMD the FP32 kernel is compiled for (bounds its
@@ -375,13 +380,14 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
         const double inv_step = 1.0 / step;
         const float eps2c = (float)(A.eps2 * inv_step * inv_step);
         const float r2c = (float)(A.r2cut * inv_step * inv_step);
-        // mass scale s (LAW 0): the power of two at or below the item's own cell mass
+        // mass scale s (LAW 0): 1 / (the power of two at or below 1 / M of the
+        // item's own cell), i.e. M <= s < 2 M
         float msc = 1.0f;
         if (LAW == 0) {
             const int64_t own = coff + (int64_t)cx * G + it.y0;
-            const float m0 = fabsf(A.l32[A.neprefix[own]].M);
-            const float p2 = __int_as_float(__float_as_int(m0) & 0x7f800000);
-            if (p2 > 0.0f && p2 < 3.0e38f) msc = p2;
+            const float r0 = fabsf(A.l32[A.neprefix[own]].rM);
+            const float p2 = __int_as_float(__float_as_int(r0) & 0x7f800000);
+            if (p2 >= 1.0e-37f && p2 <= 1.0e37f) msc = 1.0f / p2;
         }
         const float inv_msc = 1.0f / msc;
 
@@ -501,7 +507,7 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
         for (int h = 0; h < 2; ++h) {
             const int e = ne0 + lane + h * BH_WAVE;
             if (e < ne1) nxt[h] = A.l32[e];
-            else { nxt[h].fx = 1e18f; nxt[h].fy = 1e18f; nxt[h].M = 0.0f; nxt[h].kxy = 0x7fff7fff; }
+            else { nxt[h].fx = 1e18f; nxt[h].fy = 1e18f; nxt[h].rM = __builtin_huge_valf(); nxt[h].kxy = 0x7fff7fff; }
         }
         for (int base = ne0; base < ne1; base += FT) {
 #pragma unroll
@@ -512,8 +518,8 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                 const float ey = (float)(ky - oy) + en.fy;
                 float4 v;
                 if (LAW == 0) {
-                    if (en.M != 0.0f) {
-                        const float c = msc / en.M;
+                    if (fabsf(en.rM) != __builtin_huge_valf()) {   // M != 0 (NaN passes)
+                        const float c = msc * en.rM;
                         v.x = ex * c;
                         v.y = ey * c;
                         v.z = -c;
@@ -528,7 +534,7 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
                 } else {
                     v.x = ex;
                     v.y = ey;
-                    v.z = en.M;
+                    v.z = en.rM;
                     v.w = __int_as_float(en.kxy);
                 }
                 ftile[lane + h * BH_WAVE] = v;
@@ -538,7 +544,7 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
             for (int h = 0; h < 2; ++h) {
                 const int e = base + FT + lane + h * BH_WAVE;
                 if (e < ne1) nxt[h] = A.l32[e];
-                else { nxt[h].fx = 1e18f; nxt[h].fy = 1e18f; nxt[h].M = 0.0f; nxt[h].kxy = 0x7fff7fff; }
+                else { nxt[h].fx = 1e18f; nxt[h].fy = 1e18f; nxt[h].rM = __builtin_huge_valf(); nxt[h].kxy = 0x7fff7fff; }
             }
             const int b1 = base + FT;
             const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||
@@ -808,7 +814,7 @@ __global__ __launch_bounds__(64) void k_force32m(ForceArgs A)
         {
             const int e = ne0 + lane;
             if (e < ne1) nxt = A.l32[e];
-            else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+            else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.rM = __builtin_huge_valf(); nxt.kxy = 0x7fff7fff; }
         }
         for (int base = ne0; base < ne1; base += BH_WAVE) {
             {
@@ -816,9 +822,10 @@ __global__ __launch_bounds__(64) void k_force32m(ForceArgs A)
                 const float ax_ = (float)(kx - cx) + nxt.fx;
                 const float ay_ = (float)(ky - oy) + nxt.fy;
                 atile[lane] = make_float2(ax_, ay_);
-                btile[lane][0] = nxt.M * ax_;
-                btile[lane][1] = nxt.M * ay_;
-                btile[lane][2] = nxt.M;
+                const float Mv = 1.0f / nxt.rM;
+                btile[lane][0] = Mv * ax_;
+                btile[lane][1] = Mv * ay_;
+                btile[lane][2] = Mv;
                 btile[lane][3] = 0.0f;
                 ktile[lane] = nxt.kxy;
             }
@@ -826,7 +833,7 @@ __global__ __launch_bounds__(64) void k_force32m(ForceArgs A)
             {
                 const int e = base + BH_WAVE + lane;
                 if (e < ne1) nxt = A.l32[e];
-                else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.M = 0.0f; nxt.kxy = 0x7fff7fff; }
+                else { nxt.fx = 1e18f; nxt.fy = 1e18f; nxt.rM = __builtin_huge_valf(); nxt.kxy = 0x7fff7fff; }
             }
             const int b1 = base + BH_WAVE;
             const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||

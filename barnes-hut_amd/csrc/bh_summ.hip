@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_com(const SysDev* __restrict__ sys, int
             Com32 e32;
             e32.fx = (float)((X - d.min_x) / d.step - (double)kx);
             e32.fy = (float)((Y - d.min_y) / d.step - (double)ky);
-            e32.M = (float)M;
+            e32.rM = 1.0f / (float)M;
             e32.kxy = (kx << 16) | ky;
             l32[w] = e32;
         }

@@ -71,6 +71,12 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it first (python -c "
                            f"'import __graft_entry__ as g; g.build()' or make -C {_CSRC})")
+    # torch ships its own libamdhip64.  Loaded first, it satisfies the
+    # library's HIP dependency, so both share one runtime (and torch's device
+    # pointers are valid in the library's calls).  Loaded after the library,
+    # it would add a second runtime to the process, and the library's would no
+    # longer see the device.
+    _torch()
     L = ctypes.CDLL(path)
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     P = ctypes.POINTER

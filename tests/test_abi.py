@@ -62,3 +62,21 @@ def test_null_arguments_rejected_without_device():
     assert L.bh_mlp_forward(None, None, 0, None, None) == hiplib.BH_EINVAL
     q = ctypes.c_int64()
     assert L.bh_query(None, 0, ctypes.byref(q)) == hiplib.BH_EINVAL
+
+
+def test_load_binds_the_library_to_torchs_hip_runtime():
+    """hiplib.load() in a fresh process, torch imported only afterwards by the
+    caller: the process must hold one HIP runtime (torch's), not /opt/rocm's
+    beside it, or the library's calls cannot see the device."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path[:0] = sys.argv[1:]\n"
+            "from barneshut.kernels import hiplib\n"
+            "hiplib.load()\n"
+            "import torch\n"
+            "libs = sorted({l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l})\n"
+            "print('\\n'.join(libs))\n")
+    pkg = os.path.join(ROOT, "barnes-hut_amd")
+    out = subprocess.run([sys.executable, "-c", code, pkg], capture_output=True, text=True,
+                         timeout=300, check=True).stdout.split()
+    assert len(out) == 1, out

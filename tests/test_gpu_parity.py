@@ -487,3 +487,65 @@ def test_fp32_tail_split_matches_oracle(split, tail):
     np.testing.assert_array_equal(again["ax"], first["ax"])
     np.testing.assert_array_equal(again["ay"], first["ay"])
     eng.close()
+
+
+def _sweep_case(seed):
+    """A small random system: size, grid, layout and law drawn from the seed."""
+    rng = np.random.default_rng(9000 + seed)
+    kind = ("uniform", "clusters", "ties", "line", "lognormal_mass")[seed % 5]
+    n = int(rng.integers(200, 6000))
+    G = int(rng.integers(1, 48))
+    if kind == "uniform":
+        d = inputs.uniform(n, seed=seed)
+    elif kind == "clusters":
+        d = inputs.gaussian_mixture(n, int(rng.integers(1, 9)), (0.005, 0.1), seed=seed)
+    elif kind == "ties":
+        d = _ties(n, seed=seed)
+    elif kind == "line":   # zero extent in x: the square-up widens x to the y extent
+        d = inputs.uniform(n, seed=seed)
+        d["px"][:] = 0.5
+    else:                  # masses over six decades
+        d = inputs.gaussian_mixture(n, 4, (0.01, 0.1), seed=seed)
+        d["mass"] = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), n))
+    # layouts that can hold identical positions (the ties, and mixtures whose
+    # clipping can put two particles on one corner) need a softened or cut-off law
+    law = LAWS[seed % 4] if kind in ("uniform", "line") else LAWS[1 + seed % 3]
+    return kind, d, G, law
+
+
+@pytest.mark.parametrize("seed", range(15))
+def test_random_small_systems_sweep(seed):
+    """Seeded sweep over small systems (n, G, layout, law drawn per seed):
+    placement bit-exact, FP64 forces against the oracle, FP32 (with the FP64
+    recompute) against the oracle's FP64 forces."""
+    kind, d, G, law = _sweep_case(seed)
+    _, Gc, eps2, r2 = law
+    n = len(d["px"])
+    print(f"[sweep {seed}] kind={kind} n={n} G={G} law={law[0]}", flush=True)
+    eng = _forces_setup(d, G)
+    eng.check()
+    bb, key, perm, cnt, off = _oracle_placement(d, G)
+    gbb = eng.bbox()[0]
+    assert gbb[0] == bb[0] and gbb[1] == bb[1] and gbb[2] == bb[2] and gbb[4] == bb[4]
+    np.testing.assert_array_equal(eng.keys_in(), key)
+    np.testing.assert_array_equal(eng.counts(), cnt)
+    np.testing.assert_array_equal(eng.perm(), perm)
+    h, com, rx, ry = _oracle_forces(eng, G, Gc, eps2, r2)
+    eng.evaluate(Gc, eps2, r2, "fp64")
+    eng.check()
+    g = eng.fetch()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    print(f"[sweep {seed}] fp64 normwise={norm:.3e} p99={np.quantile(per, 0.99):.3e}", flush=True)
+    assert norm <= 1e-12 and np.quantile(per, 0.99) <= 1e-10
+    eng.parts["ax"].zero_()   # evaluate accumulates, as the reference does
+    eng.parts["ay"].zero_()
+    eng.evaluate(Gc, eps2, r2, "fp32")
+    eng.check()
+    g = eng.fetch()
+    norm, per = _rel(g["ax"], g["ay"], rx, ry)
+    print(f"[sweep {seed}] fp32 normwise={norm:.3e} p99={np.quantile(per, 0.99):.3e} "
+          f"max={per.max():.3e}", flush=True)
+    assert norm <= 1e-5 and np.quantile(per, 0.99) <= 1e-5
+    if r2 == 0.0:   # near r2 = r2_cut the two precisions may keep different terms
+        assert per.max() <= 1e-5
+    eng.close()

@@ -119,11 +119,11 @@ def run_path(eng, precision, steps, warmup, order_fn=None, kappa=RECOMPUTE_KAPPA
     return (t1 - t0) / steps, kms, rms, near, far
 
 
-def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7, kappa=RECOMPUTE_KAPPA):
+def force_errors(d, G, eng_cls, kappa=RECOMPUTE_KAPPA):
     """FP32 vs FP64 accelerations on the same placement: normwise, p99 and
     max per-particle relative error over all N particles (FP64 GPU path as
-    the reference), with and without the FP64 recompute, plus FP64 GPU path vs
-    the CPU oracle on a sample."""
+    the reference), with and without the FP64 recompute.  The FP64 path itself
+    is checked against the oracle by the tests, on this workload at full size."""
     import torch
     eng = eng_cls([len(d["px"])], [G])
     eng.load(d)
@@ -151,27 +151,16 @@ def force_errors(d, G, eng_cls, oracle_sample=2048, seed=7, kappa=RECOMPUTE_KAPP
         if kap > 0:
             out[name]["recomputed"] = int(eng.ctx.stats()[6])
     eng.check()
-    # FP64 GPU vs the CPU oracle on sampled particles
-    from oracle import oracle as orc
-    h = eng.fetch()
-    off = eng.offsets().astype(np.int64)
-    keys = eng.keys()
-    com = orc.summarize(off, h["px"], h["py"], h["mass"])
-    rng = np.random.default_rng(seed)
-    tg = np.sort(rng.choice(len(h["px"]), oracle_sample, replace=False))
-    rx, ry = orc.forces(G, off, keys, h["px"], h["py"], h["mass"], com, GC, EPS2, R2CUT,
-                        targets=tg)
-    g64x, g64y = a64[0].cpu().numpy()[tg], a64[1].cpu().numpy()[tg]
-    per = np.hypot(g64x - rx, g64y - ry) / np.hypot(rx, ry)
-    out["fp64_vs_oracle_sample"] = dict(n=int(oracle_sample), max=float(per.max()))
     eng.close()
     return out
 
 
-def cpu_baseline(d, G, budget_s=12.0):
+def cpu_baseline(d, G, eng_cls=None, budget_s=12.0):
     """The C restatement (oracle/) of python-bh's sequential step on this host:
     full placement, COM and tick over all N; forces on a random sample of
-    target particles (OpenMP over targets), extrapolated to N."""
+    target particles (OpenMP over targets), extrapolated to N.  With eng_cls,
+    the GPU placement of the same state is then checked against the timed
+    oracle placement (bounding box, cell keys, counts, offsets, permutation)."""
     from oracle import oracle as orc
     cores = len(os.sched_getaffinity(0))
     n = len(d["px"])
@@ -204,13 +193,29 @@ def cpu_baseline(d, G, budget_s=12.0):
     orc.tick(xx, yy, vx, vy, ax, ay, DT)
     t_tick = time.perf_counter() - tb
     t_step = (t1 - t0) + (t2 - t1) + t_force + t_tick
+    check = None
+    if eng_cls is not None:   # after the timing: the oracle's placement as the checker
+        cnt, _ = orc.counts_offsets(ks, G * G)
+        eng = eng_cls([n], [G])
+        eng.load(d)
+        eng.create_tree()
+        eng.check()
+        gbb = eng.bbox()[0]
+        check = {"n": n,
+                 "bbox": bool(gbb[0] == bb[0] and gbb[1] == bb[1] and gbb[2] == bb[2]
+                              and gbb[4] == bb[4]),
+                 "keys": bool(np.array_equal(eng.keys_in(), key)),
+                 "counts": bool(np.array_equal(eng.counts(), cnt)),
+                 "offsets": bool(np.array_equal(eng.offsets(), off)),
+                 "permutation": bool(np.array_equal(eng.perm(), perm))}
+        eng.close()
     return {"value": n / t_step / 1e6, "unit": "Mparticle-steps/s", "cores": cores,
             "kind": "port",
             "sample": (f"C restatement of the sequential step (oracle/bh_oracle.c): placement "
                        f"({t1 - t0:.2f}s, 1 thread), COM ({t2 - t1:.2f}s) and tick over all "
                        f"{n} particles; forces on {ns} random particles with {cores} OpenMP "
                        f"threads, x{n / ns:.1f} extrapolated ({t_force:.1f}s/step)"),
-            "seconds_per_step": t_step}
+            "seconds_per_step": t_step, "placement_check": check}
 
 
 # launch shapes the autotuner tries for the batched line (untimed, cached)
@@ -448,7 +453,9 @@ def main():
         out["reference_config"] = reference_config_line(3)
     if not args.skip_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(d, G)
+        cb = cpu_baseline(d, G, Engine)
+        out["placement_bit_exact"] = cb.pop("placement_check")
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 

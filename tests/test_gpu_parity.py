@@ -349,6 +349,38 @@ def test_fp64_forces_independent_of_tuning():
         eng.close()
 
 
+SHAPES = [{"p": 1, "yspan": 1, "waves_per_cu": 4, "tail_split": 1, "tail_items": 0},
+          {"p": 1, "yspan": 512, "waves_per_cu": 32, "tail_split": 16, "tail_items": 1 << 20},
+          {"p": 2, "yspan": 16, "waves_per_cu": 16, "tail_split": 4, "tail_items": 300},
+          {"p": 4, "yspan": 64, "waves_per_cu": 8, "tail_split": 1, "tail_items": 0},
+          {"p": 4, "yspan": 512, "waves_per_cu": 32, "tail_split": 8, "tail_items": 1 << 20}]
+
+
+@pytest.mark.parametrize("tun", SHAPES, ids=[f"p{t['p']}_y{t['yspan']}_w{t['waves_per_cu']}_s{t['tail_split']}"
+                                             for t in SHAPES])
+def test_fp32_forces_across_launch_shapes(tun):
+    """The FP32 kernels at 1, 2 and 4 particles per lane, with and without
+    far-field slices, against the oracle; each shape repeats bit for bit."""
+    d = inputs.gaussian_mixture(30000, 8, (0.005, 0.05), seed=41)
+    G = 64
+    eng = _forces_setup(d, G, tun)
+    eng.evaluate(1.0, 0.0, 0.0, "fp32")
+    eng.check()
+    h, com, rx, ry = _oracle_forces(eng, G, 1.0, 0.0, 0.0)
+    first = eng.fetch()
+    norm, per = _rel(first["ax"], first["ay"], rx, ry)
+    print(f"[fp32 shape {tun}] normwise={norm:.3e} p99={np.quantile(per, 0.99):.3e} "
+          f"max={per.max():.3e}", flush=True)
+    assert norm <= 1e-5 and np.quantile(per, 0.99) <= 1e-5 and per.max() <= 1e-5
+    for k in ("ax", "ay"):
+        eng.parts[k].zero_()
+    eng.evaluate(1.0, 0.0, 0.0, "fp32")
+    again = eng.fetch()
+    np.testing.assert_array_equal(again["ax"], first["ax"])
+    np.testing.assert_array_equal(again["ay"], first["ay"])
+    eng.close()
+
+
 def test_step_and_graph_match_phases():
     """bh_step and its graph replay equal the phase-by-phase calls, over
     several consecutive steps."""

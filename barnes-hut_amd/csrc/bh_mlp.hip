@@ -282,19 +282,38 @@ __global__ void k_cost_init(int32_t* sc)
     }
 }
 
-__global__ void k_cost_minmax(const float* __restrict__ cost, const int32_t* __restrict__ stats,
-                              int32_t* sc)
+// Min and max key of the finite predictions, and a flag if any prediction is
+// not finite: a grid-stride loop, a wave reduction, then one pair of atomics
+// per wave (one pair per item would serialise on the two words).
+__global__ __launch_bounds__(256) void k_cost_minmax(const float* __restrict__ cost,
+                                                     const int32_t* __restrict__ stats, int32_t* sc)
 {
     const int n = stats[ST_NITEMS];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float c = cost[i];
-    if (!isfinite(c)) {
-        atomicOr(&sc[2], 1);
-        return;
+    uint32_t lo = 0xffffffffu, hi = 0u;
+    bool bad = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float c = cost[i];
+        if (!isfinite(c)) {
+            bad = true;
+            continue;
+        }
+        const uint32_t k = f_ord(c);
+        lo = min(lo, k);
+        hi = max(hi, k);
     }
-    atomicMin((unsigned int*)&sc[0], f_ord(c));
-    atomicMax((unsigned int*)&sc[1], f_ord(c));
+#pragma unroll
+    for (int o = BH_WAVE / 2; o > 0; o >>= 1) {
+        lo = min(lo, (uint32_t)__shfl_xor((int)lo, o, BH_WAVE));
+        hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, BH_WAVE));
+    }
+    const bool any_bad = __ballot(bad) != 0ull;
+    if ((threadIdx.x & (BH_WAVE - 1)) == 0) {
+        if (lo <= hi) {
+            atomicMin((unsigned int*)&sc[0], lo);
+            atomicMax((unsigned int*)&sc[1], hi);
+        }
+        if (any_bad) atomicOr(&sc[2], 1);
+    }
 }
 
 __device__ __forceinline__ int cost_bin(float c, float mn, float mx)
@@ -365,7 +384,7 @@ static int launch_order(void* ctx, const float* cost, int32_t* order, hipStream_
     const int64_t mi = bh_ctx_max_items(ctx);
     unsigned g = (unsigned)((mi + 255) / 256);
     hipLaunchKernelGGL(k_cost_init, dim3(4), dim3(1024), 0, st, sc);
-    hipLaunchKernelGGL(k_cost_minmax, dim3(g), dim3(256), 0, st, cost, stats, sc);
+    hipLaunchKernelGGL(k_cost_minmax, dim3(g < 512u ? g : 512u), dim3(256), 0, st, cost, stats, sc);
     hipLaunchKernelGGL(k_cost_hist, dim3(g), dim3(256), 0, st, cost, stats, sc);
     hipLaunchKernelGGL(k_cost_scan, dim3(1), dim3(1024), 0, st, sc);
     hipLaunchKernelGGL(k_cost_scatter, dim3(g), dim3(256), 0, st, cost, stats, sc, order);

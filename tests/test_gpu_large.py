@@ -57,53 +57,68 @@ def test_energy_matches_oracle():
         assert got[s, 1] == pytest.approx(K, rel=1e-13)
 
 
-def _trajectory(d, G, precision, steps, model=None):
+def _trajectory(d, G, precision, steps, model=None, g=GC):
     import torch
     eng = _engine(len(d["px"]), G)
     eng.load(d)
     order = None
     if model is not None:
         order = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
-    E = [float(eng.energy(GC, 0.0)[0].sum())]
-    for _ in range(steps):
+    E = [float(eng.energy(g, 0.0)[0].sum())]
+    for s in range(steps):
         eng.create_tree()
         eng.summarize()
         eng.ctx.build_items()
         if model is not None:
             eng.ctx.learned_order(model.params, order)
-        eng.ctx.force_pass(eng.parts, GC, 0.0, 0.0, precision, order)
+        eng.ctx.force_pass(eng.parts, g, 0.0, 0.0, precision, order)
         eng.timestep(DT)
-        E.append(float(eng.energy(GC, 0.0)[0].sum()))
+        E.append(float(eng.energy(g, 0.0)[0].sum()))
+        if (s + 1) % 5 == 0:
+            print(f"[drift] {precision} n={len(d['px'])} step {s + 1}/{steps}", flush=True)
     eng.check()
     eng.close()
     return np.array(E)
 
 
 def test_energy_drift_scaled_large_mix():
-    """20 steps of the large mixture scaled to 2^18 particles: the FP32
-    learned-order path (FP64 COM and update) follows the FP64 path's energy.
-    Measured on MI355X: drift -4.5e-4 for both paths, max |E32 - E64| / |E0|
-    = 1.9e-8."""
+    """20 steps of the large mixture scaled to 2^18 particles (BH_DRIFT_LOG2N
+    sets another even power of two; the grid keeps 4 particles per cell as in
+    the 67M-particle workload): the FP32 learned-order path (FP64 COM and
+    update) follows the FP64 path's energy as closely as an exact 1e-7 change
+    of the law does.  The control is the FP64 path with G x (1 + 1e-7) (the
+    energy measured with the same G).  Without softening, close encounters
+    amplify any difference, so the gap is compared with that control rather
+    than with a fixed fraction of the drift.  Measured on MI355X at 2^18: drift
+    -4.5e-4 for both paths, max |E32 - E64| / |E0| = 9.2e-9 (control 1.1e-8);
+    at 2^20 a close encounter at step 11 separates the histories (FP32 4.0e-5
+    of |E0|, control 3.9e-5)."""
     import torch
     from barneshut import costmodel
-    n, G = 1 << 18, 256
+    log2n = int(os.environ.get("BH_DRIFT_LOG2N", "18"))
+    assert log2n % 2 == 0 and 10 <= log2n <= 22
+    n, G = 1 << log2n, 1 << ((log2n - 2) // 2)
     d = inputs.gaussian_mixture(n, 16, SIGMA, seed=71, background=0.2)
     model = costmodel.CostModel.load(MODEL, device=torch.device("cuda"))
     e64 = _trajectory(d, G, "fp64", 20)
+    e64p = _trajectory(d, G, "fp64", 20, g=GC * (1.0 + 1e-7))
     e32 = _trajectory(d, G, "fp32", 20, model)
     d64 = (e64 - e64[0]) / abs(e64[0])
     d32 = (e32 - e32[0]) / abs(e32[0])
     gap = np.abs(e32 - e64) / abs(e64[0])
+    gap_ctl = np.abs((e64p - e64p[0]) / abs(e64p[0]) - d64)
     print(f"[drift] E0={e64[0]:.9e} fp64 drift={d64[-1]:.3e} fp32 drift={d32[-1]:.3e} "
-          f"max|E32-E64|/|E0|={gap.max():.3e}", flush=True)
+          f"max|E32-E64|/|E0|={gap.max():.3e} control (G x (1+1e-7))={gap_ctl.max():.3e}",
+          flush=True)
     print("[drift] fp64 " + " ".join(f"{v:.2e}" for v in d64), flush=True)
     print("[drift] fp32 " + " ".join(f"{v:.2e}" for v in d32), flush=True)
-    assert np.isfinite(e64).all() and np.isfinite(e32).all()
+    assert np.isfinite(e64).all() and np.isfinite(e32).all() and np.isfinite(e64p).all()
     assert e32[0] == e64[0]
     # the integrator's own drift (symplectic Euler, no softening) stays small,
-    # and FP32 forces move the energy history by < 0.1 % of that drift
+    # and FP32 forces move the energy history by < 0.1 % of that drift or by
+    # no more than twice what an exact 1e-7 change of the law does
     assert np.abs(d64).max() <= 5e-3
-    assert gap.max() <= 1e-3 * np.abs(d64).max()
+    assert gap.max() <= max(1e-3 * np.abs(d64).max(), 2.0 * gap_ctl.max())
 
 
 def _sorted_under_total_order(h, keys):

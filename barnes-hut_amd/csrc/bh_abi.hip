@@ -196,8 +196,9 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     int64_t N = 0, C = 0, cols = 0, maxN = 0, NB = 0;
     for (int s = 0; s < nsys; ++s) {
         if (n[s] < 1) return fail(BH_EINVAL, "system " + std::to_string(s) + " has n < 1");
-        if (grid_dim[s] < 1 || grid_dim[s] > 46340)
-            return fail(BH_EINVAL, "system " + std::to_string(s) + " grid_dim out of [1, 46340]");
+        // the FP32 COM list packs column and row in 16 bits each (Com32::kxy)
+        if (grid_dim[s] < 1 || grid_dim[s] > 32767)
+            return fail(BH_EINVAL, "system " + std::to_string(s) + " grid_dim out of [1, 32767]");
         SysDev& d = c->sys_h[s];
         d.p_off = N;
         d.n = n[s];

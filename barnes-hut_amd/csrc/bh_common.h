@@ -65,7 +65,7 @@ struct Com64 {
 // the FP32 kernels need s / M for a power of two s, which is s * rM exactly.
 struct Com32 {
     float fx, fy, rM;
-    int32_t kxy;   // (kx << 16) | ky
+    int32_t kxy;   // (kx << 16) | ky; kx, ky < 32768 (bh_set_systems caps grid_dim)
 };
 
 // A force work item: a run of sorted particles [p0, p0 + np) that all lie in

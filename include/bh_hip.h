@@ -101,8 +101,10 @@ int bh_create(int device, void** ctx_out);
 int bh_destroy(void* ctx);
 
 /* Defines the independent systems (host arrays of length nsys): system s
- * holds n[s] particles at rows [sum_{t<s} n[t], ...) and a grid_dim[s]^2
- * grid. (Re)allocates the context's scratch. */
+ * holds n[s] >= 1 particles at rows [sum_{t<s} n[t], ...) and a
+ * grid_dim[s]^2 grid, 1 <= grid_dim[s] <= 32767 (the FP32 COM list packs a
+ * cell's column and row in 16 bits each; such a grid already needs ~170 B of
+ * per-cell state, ~180 GB). (Re)allocates the context's scratch. */
 int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* grid_dim);
 int bh_set_tuning(void* ctx, const bh_tuning* t);
 int bh_get_tuning(void* ctx, bh_tuning* t);

@@ -143,6 +143,8 @@ def test_invalid_arguments_raise():
     from barneshut.kernels.hiplib import BhError
     with pytest.raises(BhError):
         Engine([100], [0])
+    with pytest.raises(BhError, match="32767"):
+        Engine([100], [32768])     # beyond the FP32 COM list's 16-bit cell packing
     e = Engine([1000], [8])
     with pytest.raises(BhError):
         e.ctx.set_tuning(p=3)

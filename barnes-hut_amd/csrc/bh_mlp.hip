@@ -38,6 +38,7 @@ int bh_ctx_fail(int code, const char* msg);
 static_assert(OFF_B3 + 1 == BH_MLP_NPARAM, "parameter layout");
 
 #define COST_BINS 4096
+#define COST_CHUNKS 256   // item chunks of the order's histogram and scatter
 
 // ---------------------------------------------------------------- features
 // f0 log1p(np)              particles in the item
@@ -324,14 +325,25 @@ __device__ __forceinline__ int cost_bin(float c, float mn, float mx)
     return COST_BINS - 1 - b;   // descending cost
 }
 
-__global__ void k_cost_hist(const float* __restrict__ cost, const int32_t* __restrict__ stats,
-                            int32_t* sc)
+// The items are cut into gridDim.x contiguous chunks.  A block counts its
+// chunk's bins in LDS and adds only the non-zero ones to the global
+// histogram (one global atomic per item would contend on the bins).
+__global__ __launch_bounds__(1024) void k_cost_hist(const float* __restrict__ cost,
+                                                    const int32_t* __restrict__ stats, int32_t* sc)
 {
+    __shared__ int32_t h[COST_BINS];
     const int n = stats[ST_NITEMS];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || sc[2]) return;
-    float mn = f_val((uint32_t)sc[0]), mx = f_val((uint32_t)sc[1]);
-    atomicAdd(&sc[16 + cost_bin(cost[i], mn, mx)], 1);
+    if (sc[2]) return;   // written by the previous kernel: the same for every thread
+    const float mn = f_val((uint32_t)sc[0]), mx = f_val((uint32_t)sc[1]);
+    for (int b = threadIdx.x; b < COST_BINS; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int i0 = (int)blockIdx.x * per, i1 = min(n, i0 + per);
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x)
+        atomicAdd(&h[cost_bin(cost[i], mn, mx)], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < COST_BINS; b += blockDim.x)
+        if (h[b]) atomicAdd(&sc[16 + b], h[b]);
 }
 
 __global__ __launch_bounds__(1024) void k_cost_scan(int32_t* sc)
@@ -361,20 +373,39 @@ __global__ __launch_bounds__(1024) void k_cost_scan(int32_t* sc)
     if (threadIdx.x == 1023) off[COST_BINS] = base;
 }
 
-__global__ void k_cost_scatter(const float* __restrict__ cost, const int32_t* __restrict__ stats,
-                               int32_t* sc, int32_t* __restrict__ order)
+// Same chunks as k_cost_hist: a block counts its chunk's bins in LDS,
+// reserves one range per non-zero bin in the global order (one atomic per
+// bin), and places each item at its range's base plus its rank in the block.
+// Items of one bin may land in any order; each item lands exactly once.
+__global__ __launch_bounds__(1024) void k_cost_scatter(const float* __restrict__ cost,
+                                                       const int32_t* __restrict__ stats, int32_t* sc,
+                                                       int32_t* __restrict__ order)
 {
+    __shared__ int32_t h[COST_BINS];   // the chunk's count, then its base position, per bin
+    __shared__ int32_t r[COST_BINS];   // rank of the next item per bin
     const int n = stats[ST_NITEMS];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int per = (n + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int i0 = (int)blockIdx.x * per, i1 = min(n, i0 + per);
     if (sc[2]) {   // a prediction is not finite: static order
-        order[i] = i;
+        for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) order[i] = i;
         return;
     }
-    float mn = f_val((uint32_t)sc[0]), mx = f_val((uint32_t)sc[1]);
-    int b = cost_bin(cost[i], mn, mx);
-    int pos = sc[16 + COST_BINS + b] + atomicAdd(&sc[16 + 2 * COST_BINS + 1 + b], 1);
-    order[pos] = i;
+    const float mn = f_val((uint32_t)sc[0]), mx = f_val((uint32_t)sc[1]);
+    for (int b = threadIdx.x; b < COST_BINS; b += blockDim.x) {
+        h[b] = 0;
+        r[b] = 0;
+    }
+    __syncthreads();
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x)
+        atomicAdd(&h[cost_bin(cost[i], mn, mx)], 1);
+    __syncthreads();
+    for (int b = threadIdx.x; b < COST_BINS; b += blockDim.x)
+        if (h[b]) h[b] = sc[16 + COST_BINS + b] + atomicAdd(&sc[16 + 2 * COST_BINS + 1 + b], h[b]);
+    __syncthreads();
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += blockDim.x) {
+        const int b = cost_bin(cost[i], mn, mx);
+        order[h[b] + atomicAdd(&r[b], 1)] = i;
+    }
 }
 
 static int launch_order(void* ctx, const float* cost, int32_t* order, hipStream_t st)
@@ -385,9 +416,11 @@ static int launch_order(void* ctx, const float* cost, int32_t* order, hipStream_
     unsigned g = (unsigned)((mi + 255) / 256);
     hipLaunchKernelGGL(k_cost_init, dim3(4), dim3(1024), 0, st, sc);
     hipLaunchKernelGGL(k_cost_minmax, dim3(g < 512u ? g : 512u), dim3(256), 0, st, cost, stats, sc);
-    hipLaunchKernelGGL(k_cost_hist, dim3(g), dim3(256), 0, st, cost, stats, sc);
+    // COST_CHUNKS blocks of 1024 threads, one per chunk of the items
+    // (k_cost_hist and k_cost_scatter must cut the same chunks)
+    hipLaunchKernelGGL(k_cost_hist, dim3(COST_CHUNKS), dim3(1024), 0, st, cost, stats, sc);
     hipLaunchKernelGGL(k_cost_scan, dim3(1), dim3(1024), 0, st, sc);
-    hipLaunchKernelGGL(k_cost_scatter, dim3(g), dim3(256), 0, st, cost, stats, sc, order);
+    hipLaunchKernelGGL(k_cost_scatter, dim3(COST_CHUNKS), dim3(1024), 0, st, cost, stats, sc, order);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return bh_ctx_fail(BH_EHIP, hipGetErrorString(e));
     return BH_OK;

@@ -126,9 +126,22 @@ def sort_perm(key, fields):
     return perm
 
 
+def _check_keys(key, ncells):
+    """The C routines index per-cell arrays by key: refuse keys outside [0, ncells)."""
+    if len(key) and (int(key.min()) < 0 or int(key.max()) >= ncells):
+        raise ValueError(f"cell keys must lie in [0, {ncells})")
+
+
+def _check_offsets(offsets, n):
+    """Exclusive per-cell offsets: start at 0, non-decreasing, end at n."""
+    if len(offsets) < 1 or offsets[0] != 0 or offsets[-1] != n or np.any(np.diff(offsets) < 0):
+        raise ValueError(f"offsets must rise from 0 to {n}")
+
+
 def counts_offsets(key_sorted, ncells):
     """Per-cell counts and exclusive offsets (ncells+1) in x-major key order."""
     key_sorted = np.ascontiguousarray(key_sorted, dtype=np.int32)
+    _check_keys(key_sorted, ncells)
     c = np.empty(ncells, np.int64)
     o = np.empty(ncells + 1, np.int64)
     lib().bho_counts(len(key_sorted), _i32(key_sorted), int(ncells), _i64(c), _i64(o))
@@ -139,6 +152,9 @@ def summarize(offsets, x, y, m):
     """cloud.py:9-17 per cell -> com (ncells, 3) = (X, Y, M); empty -> 0."""
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
     x, y, m = _f64c(x), _f64c(y), _f64c(m)
+    _check_offsets(offsets, len(x))
+    if not len(x) == len(y) == len(m):
+        raise ValueError("x, y and m must have one entry per particle")
     ncells = len(offsets) - 1
     com = np.empty((ncells, 3))
     lib().bho_summarize(ncells, _i64(offsets), _d(x), _d(y), _d(m), _d(com))
@@ -153,8 +169,16 @@ def forces(G, offsets, key_sorted, x, y, m, com, Gc=1.0, eps2=0.0, r2_cut=0.0,
     x, y, m = _f64c(x), _f64c(y), _f64c(m)
     com = _f64c(com)
     n = len(x)
+    G = int(G)
+    if not len(y) == len(m) == len(key_sorted) == n or len(offsets) != G * G + 1 \
+            or com.size != 3 * G * G:
+        raise ValueError("forces: arrays must match n particles and G*G cells")
+    _check_offsets(offsets, n)
+    _check_keys(key_sorted, G * G)
     if targets is not None:
         targets = np.ascontiguousarray(targets, dtype=np.int64)
+        if len(targets) and (int(targets.min()) < 0 or int(targets.max()) >= n):
+            raise ValueError(f"targets must lie in [0, {n})")
         nt = len(targets)
         tp = _i64(targets)
     else:

@@ -306,3 +306,27 @@ def test_energy_gradient_is_the_force_law():
             arr[i] += h
             grad = (up - dn) / (2 * h)
             assert -grad == pytest.approx(m[i] * acc[i], rel=1e-6, abs=1e-8)
+
+
+def test_oracle_refuses_inconsistent_inputs():
+    """The C restatement indexes per-cell arrays by key and offset: its
+    wrappers refuse keys, offsets, shapes and targets that would run out of
+    bounds (found with an AddressSanitizer build of the oracle)."""
+    with pytest.raises(ValueError):
+        orc.counts_offsets(np.array([0, 5], np.int32), 4)
+    with pytest.raises(ValueError):
+        orc.counts_offsets(np.array([-1, 0], np.int32), 4)
+    x = np.array([0.1, 0.9]); m = np.ones(2)
+    with pytest.raises(ValueError):
+        orc.summarize(np.array([0, 1, 3]), x, x, m)          # offsets end past n
+    key = np.array([0, 3], np.int32)
+    cnt, off = orc.counts_offsets(key, 4)
+    com = orc.summarize(off, x, x, m)
+    with pytest.raises(ValueError):
+        orc.forces(2, off, np.array([0, 4], np.int32), x, x, m, com)   # key past G*G
+    with pytest.raises(ValueError):
+        orc.forces(3, off, key, x, x, m, com)                 # offsets of another grid
+    with pytest.raises(ValueError):
+        orc.forces(2, off, key, x, x, m, com, targets=np.array([2]))
+    ax, ay = orc.forces(2, off, key, x, x, m, com)
+    assert np.isfinite(ax).all() and np.isfinite(ay).all()

@@ -7,7 +7,10 @@ is timed with HIP events on the engine's current state (after create_tree and
 summarize).  For FP32 paths a candidate is admitted only if its accelerations
 stay within `max_rel_err` (normwise) of an FP64 pass on the same state: the
 item span sets the FP32 origin distance, so it trades accuracy for occupancy.
-Results are cached per (device, n, grid_dim, precision) in a JSON file.
+Results are cached in a JSON file per device, system sizes and grids (all of
+them, hashed), precision, law, threshold and library build; an FP32 shape
+taken from the cache is checked against an FP64 pass on the current state
+again, and tuned afresh if it no longer passes.
 """
 from __future__ import annotations
 
@@ -20,20 +23,43 @@ DEFAULT_CANDIDATES = dict(p=(1, 2, 4), yspan=(128, 512), waves_per_cu=(16, 32),
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "bh_mi355x_autotune.json")
 
 
-def _key(eng, precision):
-    """Device, systems, precision and the library build (a rebuilt kernel is
-    tuned again)."""
+def _key(eng, precision, law, max_rel_err):
+    """Device, every system's size and grid (hashed), precision, law, the
+    admissibility threshold and the library build (a rebuilt kernel is tuned
+    again)."""
+    import hashlib
+
+    import numpy as np
     import torch
     from .kernels import hiplib
     name = torch.cuda.get_device_name(eng.dev)
-    n = ",".join(str(int(v)) for v in eng.n[:8])
-    g = ",".join(str(int(v)) for v in eng.grid_dim[:8])
+    h = hashlib.sha1()
+    h.update(np.ascontiguousarray(eng.n, dtype=np.int64).tobytes())
+    h.update(np.ascontiguousarray(eng.grid_dim, dtype=np.int32).tobytes())
     try:
         st = os.stat(hiplib.LIB_PATH)
         build = f"{st.st_size}:{int(st.st_mtime)}"
     except OSError:
         build = "?"
-    return f"{name}|S={len(eng.n)}|n={n}|G={g}|{precision}|lib={build}"
+    G, eps2, r2 = (float(v) for v in law)
+    return (f"{name}|S={len(eng.n)}|N={int(eng.n.sum())}|systems={h.hexdigest()[:16]}|"
+            f"{precision}|law={G!r},{eps2!r},{r2!r}|tol={float(max_rel_err)!r}|lib={build}")
+
+
+def _fp64_reference(eng, law):
+    import torch
+    eng.ctx.build_items()
+    eng.parts["ax"].zero_()
+    eng.parts["ay"].zero_()
+    eng.ctx.force_pass(eng.parts, *law, "fp64")
+    ref = (eng.parts["ax"].clone(), eng.parts["ay"].clone())
+    return ref, torch.sqrt((ref[0] ** 2 + ref[1] ** 2).sum())
+
+
+def _rel_err(eng, ref, den):
+    import torch
+    num = torch.sqrt(((eng.parts["ax"] - ref[0]) ** 2 + (eng.parts["ay"] - ref[1]) ** 2).sum())
+    return float(num / den)
 
 
 def _time_force(eng, law, precision, order=None, reps=2):
@@ -56,43 +82,50 @@ def tune(eng, law=(1.0, 0.0, 0.0), precision="fp32", candidates=None, max_rel_er
          reps=2, cache=CACHE, use_cache=True, log=None):
     """Pick the fastest admissible tuning for `eng`'s current state and set it.
     law = (G, eps2, r2_cut).  Returns dict(tuning, ms, table)."""
-    import torch
-    key = _key(eng, precision)
+    key = _key(eng, precision, law, max_rel_err)
+    fp32 = not precision.startswith("fp64")
+    saved = {k: eng.parts[k].clone() for k in ("ax", "ay")}
+    ref = den = None
+    if fp32:
+        ref, den = _fp64_reference(eng, law)
     if use_cache and cache and os.path.exists(cache):
+        hit = None
         try:
             with open(cache) as f:
                 hit = json.load(f).get(key)
-            if hit:
-                eng.ctx.set_tuning(**hit["tuning"])
-                return hit
         except (OSError, ValueError):
-            pass
+            hit = None
+        if hit:
+            # a cached FP32 shape is admitted again on this state: one FP32 pass
+            # against the FP64 pass above
+            eng.ctx.set_tuning(**hit["tuning"])
+            eng.ctx.build_items()
+            ok = True
+            if fp32:
+                _time_force(eng, law, precision, reps=1)
+                hit["rel_err"] = _rel_err(eng, ref, den)
+                ok = hit["rel_err"] <= max_rel_err
+                if log:
+                    log(f"tune cache hit {hit['tuning']} err {hit['rel_err']:.2e}"
+                        f"{'' if ok else ' (rejected: tuning again)'}")
+            if ok:
+                for k in ("ax", "ay"):
+                    eng.parts[k].copy_(saved[k])
+                hit["cached"] = True
+                return hit
     cand = dict(DEFAULT_CANDIDATES)
     if candidates:
         cand.update(candidates)
-    saved = {k: eng.parts[k].clone() for k in ("ax", "ay")}
-    ref = None
-    if precision != "fp64":
-        eng.ctx.build_items()
-        eng.parts["ax"].zero_()
-        eng.parts["ay"].zero_()
-        eng.ctx.force_pass(eng.parts, *law, "fp64")
-        ref = (eng.parts["ax"].clone(), eng.parts["ay"].clone())
-        den = torch.sqrt((ref[0] ** 2 + ref[1] ** 2).sum())
     table = []
     best = None
-    splits = cand["tail_split"] if precision != "fp64" else (1,)
+    splits = cand["tail_split"] if fp32 else (1,)
     for p, ys, w, sp in itertools.product(cand["p"], cand["yspan"], cand["waves_per_cu"], splits):
         tun = dict(p=int(p), yspan=int(ys), waves_per_cu=int(w), tail_split=int(sp),
                    tail_items=(1 << 20) if sp > 1 else 0)
         eng.ctx.set_tuning(**tun)
         eng.ctx.build_items()
         ms = _time_force(eng, law, precision, reps=reps)
-        err = 0.0
-        if ref is not None:
-            num = torch.sqrt(((eng.parts["ax"] - ref[0]) ** 2
-                              + (eng.parts["ay"] - ref[1]) ** 2).sum())
-            err = float(num / den)
+        err = _rel_err(eng, ref, den) if ref is not None else 0.0
         ok = err <= max_rel_err
         table.append(dict(tuning=tun, ms=ms, rel_err=err, admitted=ok))
         if log:

@@ -16,10 +16,12 @@ _CSRC = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abs
 LIB_PATH = os.path.join(_CSRC, "libbh_hip.so")
 
 BH_OK, BH_EINVAL, BH_EHIP, BH_EDEVICE, BH_ESTATE = 0, 1, 2, 3, 4
-PREC_FP64, PREC_FP32_KAHAN, PREC_FP32_KAHAN_TERM, PREC_FP32_PLAIN, PREC_FP32_MFMA = 0, 1, 2, 3, 4
+(PREC_FP64, PREC_FP32_KAHAN, PREC_FP32_KAHAN_TERM, PREC_FP32_PLAIN, PREC_FP32_MFMA,
+ PREC_FP64_IEEE) = range(6)
 PRECISIONS = {"fp64": PREC_FP64, "fp32": PREC_FP32_KAHAN, "fp32_kahan": PREC_FP32_KAHAN,
               "fp32_kahan_term": PREC_FP32_KAHAN_TERM, "fp32_plain": PREC_FP32_PLAIN,
-              "fp32_mfma": PREC_FP32_MFMA}
+              "fp32_mfma": PREC_FP32_MFMA, "fp64_ieee": PREC_FP64_IEEE}
+FP64_PRECISIONS = ("fp64", "fp64_ieee")
 (EXPORT_COUNTS, EXPORT_OFFSETS, EXPORT_NEPREFIX, EXPORT_PERM, EXPORT_KEYS, EXPORT_COM,
  EXPORT_BBOX, EXPORT_ITEMS, EXPORT_COST, EXPORT_KEYS_IN) = range(10)
 MLP_NPARAM = 1377
@@ -148,21 +150,45 @@ def stream_handle(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def particles_struct(d):
-    """BhParticles from a dict of device tensors px, py, mass, vx, vy, ax, ay, pid."""
+def particles_struct(d, n=None):
+    """BhParticles from a dict of device tensors px, py, mass, vx, vy, ax, ay,
+    pid (each of n elements when n is given)."""
     for k in ("px", "py", "mass", "vx", "vy", "ax", "ay"):
         t = d[k]
         if t.dtype != _torch().float64 or not t.is_cuda or not t.is_contiguous():
             raise ValueError(f"{k} must be a contiguous float64 device tensor")
+        if n is not None and t.numel() != n:
+            raise ValueError(f"{k} has {t.numel()} elements, the context holds {n} particles")
     t = d["pid"]
     if t.dtype != _torch().int32 or not t.is_cuda or not t.is_contiguous():
         raise ValueError("pid must be a contiguous int32 device tensor")
+    if n is not None and t.numel() != n:
+        raise ValueError(f"pid has {t.numel()} elements, the context holds {n} particles")
     return BhParticles(*[d[k].data_ptr() for k in ("px", "py", "mass", "vx", "vy", "ax", "ay",
                                                    "pid")])
 
 
+def device_buffer(t, name, dtype, min_numel, exact=False, optional=False):
+    """Checks a device tensor handed to the library as a raw pointer: dtype,
+    contiguity, and at least (or exactly) min_numel elements."""
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name} is required")
+    torch = _torch()
+    want = getattr(torch, dtype) if isinstance(dtype, str) else dtype
+    if t.dtype != want or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous {want} device tensor")
+    n = t.numel()
+    if (n != min_numel) if exact else (n < min_numel):
+        raise ValueError(f"{name} has {n} elements, needs {'exactly' if exact else 'at least'} "
+                         f"{min_numel}")
+
+
 class Context:
-    """Owns one bh_ctx (device scratch for a set of independent systems)."""
+    """Owns one bh_ctx (device scratch for a set of independent systems).
+    Tensors passed as raw pointers are checked against the context's sizes
+    (particles, work-item capacity) before any call."""
 
     def __init__(self, device: int = 0):
         self.L = load()
@@ -171,6 +197,8 @@ class Context:
         self.h = h
         self.device = device
         self.n_sys = 0
+        self.N = None
+        self.max_items = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -194,6 +222,17 @@ class Context:
         self.n_sys = len(n)
         self.n = n.copy()
         self.grid_dim = g.copy()
+        self.N = int(n.sum())
+        self.max_items = self.query(4)
+
+    def _parts(self, d):
+        return particles_struct(d, self.N)
+
+    def _order(self, order, optional=True):
+        device_buffer(order, "order", "int32", self.max_items, optional=optional)
+
+    def _params(self, params):
+        device_buffer(params, "params", "float32", MLP_NPARAM, exact=True)
 
     def set_tuning(self, p=2, yspan=512, waves_per_cu=32, tail_split=16, tail_items=1 << 20):
         t = BhTuning(int(p), int(yspan), int(waves_per_cu), int(tail_split), int(tail_items))
@@ -216,7 +255,7 @@ class Context:
         check(self.L.bh_set_recompute(self.h, float(kappa), 1 if deferred else 0))
 
     def recompute(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, stream=None):
-        p = particles_struct(parts)
+        p = self._parts(parts)
         law = BhLaw(float(G), float(eps2), float(r2_cut))
         check(self.L.bh_recompute(self.h, ctypes.byref(p), ctypes.byref(law),
                                   stream_handle(stream)))
@@ -228,19 +267,19 @@ class Context:
 
     # ---- phases
     def place(self, parts, stream=None):
-        p = particles_struct(parts)
+        p = self._parts(parts)
         check(self.L.bh_place_particles(self.h, ctypes.byref(p), stream_handle(stream)))
 
     def cumm(self, stream=None):
         check(self.L.bh_calculate_box_cumm(self.h, stream_handle(stream)))
 
     def sort(self, src, dst, stream=None):
-        a, b = particles_struct(src), particles_struct(dst)
+        a, b = self._parts(src), self._parts(dst)
         check(self.L.bh_sort_particles(self.h, ctypes.byref(a), ctypes.byref(b),
                                        stream_handle(stream)))
 
     def summarize(self, parts, stream=None):
-        p = particles_struct(parts)
+        p = self._parts(parts)
         check(self.L.bh_summarize(self.h, ctypes.byref(p), stream_handle(stream)))
 
     def build_items(self, stream=None):
@@ -248,7 +287,8 @@ class Context:
 
     def evaluate(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", order=None,
                  stream=None):
-        p = particles_struct(parts)
+        p = self._parts(parts)
+        self._order(order)
         law = BhLaw(float(G), float(eps2), float(r2_cut))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         check(self.L.bh_evaluate_boxes(self.h, ctypes.byref(p), ctypes.byref(law), prec,
@@ -256,19 +296,20 @@ class Context:
 
     def force_pass(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", order=None,
                    stream=None):
-        p = particles_struct(parts)
+        p = self._parts(parts)
+        self._order(order)
         law = BhLaw(float(G), float(eps2), float(r2_cut))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         check(self.L.bh_force_pass(self.h, ctypes.byref(p), ctypes.byref(law), prec, ptr(order),
                                    stream_handle(stream)))
 
     def tick(self, parts, dt, stream=None):
-        p = particles_struct(parts)
+        p = self._parts(parts)
         check(self.L.bh_tick_particles(self.h, ctypes.byref(p), float(dt), stream_handle(stream)))
 
     def step(self, a, b, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", dt=0.0, do_tick=True,
              graph=False, stream=None):
-        pa, pb = particles_struct(a), particles_struct(b)
+        pa, pb = self._parts(a), self._parts(b)
         law = BhLaw(float(G), float(eps2), float(r2_cut))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         fn = self.L.bh_step_graph if graph else self.L.bh_step
@@ -277,7 +318,9 @@ class Context:
 
     def step_learned(self, a, b, params, order, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp32",
                      dt=0.0, do_tick=True, graph=False, stream=None):
-        pa, pb = particles_struct(a), particles_struct(b)
+        pa, pb = self._parts(a), self._parts(b)
+        self._params(params)
+        self._order(order, optional=False)
         law = BhLaw(float(G), float(eps2), float(r2_cut))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         check(self.L.bh_step_learned(self.h, ctypes.byref(pa), ctypes.byref(pb), ctypes.byref(law),
@@ -314,7 +357,7 @@ class Context:
 
     def energy(self, parts, G=1.0, eps2=0.0, stream=None):
         """[(U_s, K_s)] per system: direct FP64 potential of the law and kinetic energy."""
-        p = particles_struct(parts)
+        p = self._parts(parts)
         out = np.zeros(2 * self.n_sys, np.float64)
         check(self.L.bh_energy(self.h, ctypes.byref(p), float(G), float(eps2),
                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -323,31 +366,55 @@ class Context:
 
     def interactions_async(self, dev_out, stream=None):
         """dev_out: device int64 tensor [2*S+2] (near counts include self pairs)."""
+        device_buffer(dev_out, "dev_out", "int64", 2 * self.n_sys + 2)
         check(self.L.bh_interactions_async(self.h, ptr(dev_out), stream_handle(stream)))
 
     # ---- cost model
     def item_features(self, feat, stream=None):
+        """feat: device float32 of at least 8 x the work-item capacity."""
+        device_buffer(feat, "feat", "float32", MLP_NFEAT * self.max_items)
         check(self.L.bh_item_features(self.h, ptr(feat), stream_handle(stream)))
 
     def order_by_cost(self, cost, order, stream=None):
+        device_buffer(cost, "cost", "float32", self.max_items)
+        self._order(order, optional=False)
         check(self.L.bh_order_by_cost(self.h, ptr(cost), ptr(order), stream_handle(stream)))
 
     def learned_order(self, params, order, stream=None):
+        self._params(params)
+        self._order(order, optional=False)
         check(self.L.bh_learned_order(self.h, ptr(params), ptr(order), stream_handle(stream)))
 
 
+def _mlp_rows(x, name="x"):
+    if x.dim() != 2 or x.shape[1] != MLP_NFEAT:
+        raise ValueError(f"{name} must be [n, {MLP_NFEAT}]")
+    device_buffer(x, name, "float32", x.shape[0] * MLP_NFEAT, exact=True)
+    return int(x.shape[0])
+
+
 def mlp_forward(params, x, y, stream=None):
+    device_buffer(params, "params", "float32", MLP_NPARAM, exact=True)
+    n = _mlp_rows(x)
+    device_buffer(y, "y", "float32", n)
     L = load()
     check(L.bh_mlp_forward(ptr(params), ptr(x), int(x.shape[0]), ptr(y), stream_handle(stream)))
 
 
 def mlp_backward(params, x, target, grad, loss=None, stream=None):
+    device_buffer(params, "params", "float32", MLP_NPARAM, exact=True)
+    n = _mlp_rows(x)
+    device_buffer(target, "target", "float32", n)
+    device_buffer(grad, "grad", "float32", MLP_NPARAM, exact=True)
+    device_buffer(loss, "loss", "float32", 1, optional=True)
     L = load()
     check(L.bh_mlp_backward(ptr(params), ptr(x), ptr(target), int(x.shape[0]), ptr(grad),
                             ptr(loss), stream_handle(stream)))
 
 
 def mlp_adam(params, grad, m, v, t, lr, stream=None):
+    for name, b in (("params", params), ("grad", grad), ("m", m), ("v", v)):
+        device_buffer(b, name, "float32", MLP_NPARAM, exact=True)
     L = load()
     check(L.bh_mlp_adam(ptr(params), ptr(grad), ptr(m), ptr(v), int(t), float(lr),
                         stream_handle(stream)))

@@ -138,6 +138,138 @@ static inline unsigned grid_for(int64_t n, int bs, int64_t cap = 1 << 20)
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, bs), cap));
 }
 
+// Checks the system sizes and lays the systems out in the global particle,
+// cell, column and recompute-block index spaces (host only, nothing changes
+// on failure).
+static int layout_systems(int32_t nsys, const int64_t* n, const int32_t* grid_dim,
+                          std::vector<SysDev>& sh, int64_t& N, int64_t& C, int64_t& cols,
+                          int64_t& maxN, int64_t& NB)
+{
+    sh.assign(nsys, SysDev{});
+    N = C = cols = maxN = NB = 0;
+    for (int s = 0; s < nsys; ++s) {
+        if (n[s] < 1) return fail(BH_EINVAL, "system " + std::to_string(s) + " has n < 1");
+        // the FP32 COM list packs column and row in 16 bits each (Com32::kxy)
+        if (grid_dim[s] < 1 || grid_dim[s] > 32767)
+            return fail(BH_EINVAL, "system " + std::to_string(s) + " grid_dim out of [1, 32767]");
+        SysDev& d = sh[s];
+        d.p_off = N;
+        d.n = n[s];
+        d.c_off = C;
+        d.col_off = cols;
+        d.G = grid_dim[s];
+        d.Gb = (int32_t)cdiv(grid_dim[s], RC_BLOCK);
+        d.b_off = NB;
+        NB += (int64_t)d.Gb * d.Gb;
+        N += n[s];
+        C += (int64_t)grid_dim[s] * grid_dim[s];
+        cols += grid_dim[s];
+        maxN = std::max(maxN, n[s]);
+        if (N >= (1ll << 31) - 1 || C >= (1ll << 31) - 1)
+            return fail(BH_EINVAL, "total particles and cells must each be < 2^31");
+    }
+    return BH_OK;
+}
+
+static void set_sizes(Ctx* c, int32_t nsys, int64_t N, int64_t C, int64_t cols, int64_t maxN,
+                      int64_t NB)
+{
+    c->S = nsys;
+    c->N = N;
+    c->C = C;
+    c->ncols = cols;
+    c->maxN = maxN;
+    c->nblocks = NB;
+    c->max_items = C + cols + 2 * cdiv(N, 64) + 64;
+}
+
+// Every device buffer of a configured context with its element count, for the
+// sizes set by set_sizes.  bh_set_systems gives each buffer its own
+// allocation; the allocation-free step carves them out of one static array.
+template <typename F>
+static int for_each_buffer(Ctx* c, F&& f)
+{
+    const int64_t N = c->N, C = c->C, cols = c->ncols, NB = c->nblocks, S = c->S;
+    const int64_t nb = std::max(cdiv(C, SCAN_TILE), cdiv(cols, SCAN_TILE)) + 1;
+    int r = BH_OK;
+#define B(p, cnt) \
+    if ((r = f(c->p, (int64_t)(cnt))) != BH_OK) return r
+    B(sys, S);
+    B(sys_of, N);
+    B(bb, 4 * S);
+    B(key, N);
+    B(key_sorted, N);
+    B(counts, C);
+    B(cursor, C);
+    B(offsets, C + 1);
+    B(neprefix, C + 1);
+    B(ne_list, C);
+    B(idx_a, N);
+    B(idx_b, N);
+    B(pxkey, N);
+    B(pykey, N);
+    B(perm, N);
+    B(bsum, nb);
+    B(bne, nb);
+    B(stats, ST_COUNT);
+    B(com_dense, 3 * C);
+    B(l64, C);
+    B(l32, C);
+    B(items, c->max_items);
+    B(col_items, cols);
+    B(col_off, cols + 1);
+    B(cost_log, c->max_items);
+    B(inter, 2 * S + 2);
+    B(cost_scratch, 16 + 3 * 4096 + 1);
+    B(feat, BH_MLP_NFEAT * c->max_items);
+    B(pred, c->max_items);
+    B(rc_a32, 3 * N);
+    B(rc_list, N);
+    B(rc_cnt, S);
+    B(rc_gofs, S + 1);
+    B(rc_part, (int64_t)RC_PART_CAP * 2 * RC_GROUP);
+    B(rc_tiles, NB * (RC_BLOCK / RC_TILE) * (RC_BLOCK / RC_TILE));
+    B(rc_blk, NB);
+    B(rc_bfar, NB);
+#undef B
+    return BH_OK;
+}
+
+// Back to the state of a fresh context (S = 0: the phase calls refuse with
+// BH_ESTATE).  The buffers must already be released (free_all).
+static void unconfigure(Ctx* c)
+{
+    (void)for_each_buffer(c, [](auto*& p, int64_t) {
+        p = nullptr;
+        return BH_OK;
+    });
+    set_sizes(c, 0, 0, 0, 0, 0, 0);
+    c->max_items = 0;
+    c->sys_h.clear();
+    c->energy_buf = nullptr;
+    c->rc_ready = 0;
+}
+
+// Allocates and initialises the buffers for sizes already validated by
+// layout_systems (c->sys_h holds the layout).
+static int configure(Ctx* c, int32_t nsys, int64_t N, int64_t C, int64_t cols, int64_t maxN,
+                     int64_t NB)
+{
+    set_sizes(c, nsys, N, C, cols, maxN, NB);
+    int r = for_each_buffer(c, [c](auto*& p, int64_t cnt) { return dalloc(c, &p, cnt); });
+    if (r != BH_OK) return r;
+    HIP_TRY(hipMemcpy(c->sys, c->sys_h.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice));
+    std::vector<int32_t> so(N);
+    for (int s = 0; s < nsys; ++s)
+        std::fill(so.begin() + c->sys_h[s].p_off, so.begin() + c->sys_h[s].p_off + c->sys_h[s].n, s);
+    HIP_TRY(hipMemcpy(c->sys_of, so.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(c->stats, 0, sizeof(int32_t) * ST_COUNT));
+    HIP_TRY(hipMemset(c->offsets, 0, sizeof(int32_t) * (C + 1)));
+    HIP_TRY(hipMemset(c->neprefix, 0, sizeof(int32_t) * (C + 1)));
+    HIP_TRY(hipDeviceSynchronize());
+    return BH_OK;
+}
+
 extern "C" {
 
 const char* bh_last_error(void) { return g_err.c_str(); }
@@ -187,93 +319,29 @@ int bh_set_systems(void* ctx, int32_t nsys, const int64_t* n, const int32_t* gri
     Ctx* c = (Ctx*)ctx;
     if (!c) return fail(BH_EINVAL, "ctx is NULL");
     if (nsys <= 0 || !n || !grid_dim) return fail(BH_EINVAL, "need nsys >= 1 and host arrays");
+    // validate and lay out everything before the old state is released, so a
+    // refused call leaves the context as it was
+    std::vector<SysDev> sh;
+    int64_t N = 0, C = 0, cols = 0, maxN = 0, NB = 0;
+    int rl = layout_systems(nsys, n, grid_dim, sh, N, C, cols, maxN, NB);
+    if (rl != BH_OK) return rl;
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipDeviceSynchronize());
+    // from here on a failure leaves the context unconfigured (S = 0: every
+    // phase call returns BH_ESTATE), never with pointers into freed memory
     free_all(c);
-    c->energy_buf = nullptr;
-    c->rc_ready = 0;
-    c->sys_h.assign(nsys, SysDev{});
-    int64_t N = 0, C = 0, cols = 0, maxN = 0, NB = 0;
-    for (int s = 0; s < nsys; ++s) {
-        if (n[s] < 1) return fail(BH_EINVAL, "system " + std::to_string(s) + " has n < 1");
-        // the FP32 COM list packs column and row in 16 bits each (Com32::kxy)
-        if (grid_dim[s] < 1 || grid_dim[s] > 32767)
-            return fail(BH_EINVAL, "system " + std::to_string(s) + " grid_dim out of [1, 32767]");
-        SysDev& d = c->sys_h[s];
-        d.p_off = N;
-        d.n = n[s];
-        d.c_off = C;
-        d.col_off = cols;
-        d.G = grid_dim[s];
-        d.Gb = (int32_t)cdiv(grid_dim[s], RC_BLOCK);
-        d.b_off = NB;
-        NB += (int64_t)d.Gb * d.Gb;
-        N += n[s];
-        C += (int64_t)grid_dim[s] * grid_dim[s];
-        cols += grid_dim[s];
-        maxN = std::max(maxN, n[s]);
+    unconfigure(c);
+    c->sys_h = sh;
+    int r = configure(c, nsys, N, C, cols, maxN, NB);
+    if (r == BH_OK) r = ensure_part(c);
+    if (r != BH_OK) {
+        const std::string msg = g_err;
+        (void)hipDeviceSynchronize();
+        free_all(c);
+        unconfigure(c);
+        return fail(r, msg);
     }
-    if (N >= (1ll << 31) - 1 || C >= (1ll << 31) - 1)
-        return fail(BH_EINVAL, "total particles and cells must each be < 2^31");
-    c->S = nsys;
-    c->N = N;
-    c->C = C;
-    c->ncols = cols;
-    c->maxN = maxN;
-    c->nblocks = NB;
-    c->max_items = C + cols + 2 * cdiv(N, 64) + 64;
-    int64_t nb = std::max(cdiv(C, SCAN_TILE), cdiv(cols, SCAN_TILE)) + 1;
-    int r = BH_OK;
-#define A(p, cnt) \
-    if ((r = dalloc(c, &c->p, (cnt))) != BH_OK) return r
-    A(sys, nsys);
-    A(sys_of, N);
-    A(bb, 4 * (int64_t)nsys);
-    A(key, N);
-    A(key_sorted, N);
-    A(counts, C);
-    A(cursor, C);
-    A(offsets, C + 1);
-    A(neprefix, C + 1);
-    A(ne_list, C);
-    A(idx_a, N);
-    A(idx_b, N);
-    A(pxkey, N);
-    A(pykey, N);
-    A(perm, N);
-    A(bsum, nb);
-    A(bne, nb);
-    A(stats, ST_COUNT);
-    A(com_dense, 3 * C);
-    A(l64, C);
-    A(l32, C);
-    A(items, c->max_items);
-    A(col_items, cols);
-    A(col_off, cols + 1);
-    A(cost_log, c->max_items);
-    A(inter, 2 * (int64_t)nsys + 2);
-    A(cost_scratch, 16 + 3 * 4096 + 1);
-    A(feat, BH_MLP_NFEAT * c->max_items);
-    A(pred, c->max_items);
-    A(rc_a32, 3 * N);
-    A(rc_list, N);
-    A(rc_cnt, nsys);
-    A(rc_gofs, nsys + 1);
-    A(rc_part, (int64_t)RC_PART_CAP * 2 * RC_GROUP);
-    A(rc_tiles, NB * (RC_BLOCK / RC_TILE) * (RC_BLOCK / RC_TILE));
-    A(rc_blk, NB);
-    A(rc_bfar, NB);
-#undef A
-    HIP_TRY(hipMemcpy(c->sys, c->sys_h.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice));
-    std::vector<int32_t> so(N);
-    for (int s = 0; s < nsys; ++s)
-        std::fill(so.begin() + c->sys_h[s].p_off, so.begin() + c->sys_h[s].p_off + c->sys_h[s].n, s);
-    HIP_TRY(hipMemcpy(c->sys_of, so.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemset(c->stats, 0, sizeof(int32_t) * ST_COUNT));
-    HIP_TRY(hipMemset(c->offsets, 0, sizeof(int32_t) * (C + 1)));
-    HIP_TRY(hipMemset(c->neprefix, 0, sizeof(int32_t) * (C + 1)));
-    HIP_TRY(hipDeviceSynchronize());
-    return ensure_part(c);
+    return BH_OK;
 }
 
 // Scratch of the split force pass: tail_split planes of N x 2 doubles.
@@ -525,7 +593,7 @@ int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32
     int r = check_parts(p, "particles");
     if (r) return r;
     if (!law) return fail(BH_EINVAL, "law is NULL");
-    if (precision < 0 || precision > 4) return fail(BH_EINVAL, "precision must be in [0, 4]");
+    if (precision < 0 || precision > 5) return fail(BH_EINVAL, "precision must be in [0, 5]");
     if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
         return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
     if (!order) {
@@ -542,7 +610,7 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     int r = check_parts(p, "particles");
     if (r) return r;
     if (!law) return fail(BH_EINVAL, "law is NULL");
-    if (precision < 0 || precision > 4) return fail(BH_EINVAL, "precision must be in [0, 4]");
+    if (precision < 0 || precision > 5) return fail(BH_EINVAL, "precision must be in [0, 5]");
     if (!(law->eps2 >= 0.0) || !(law->r2_cut >= 0.0) || !std::isfinite(law->G))
         return fail(BH_EINVAL, "law needs finite G, eps2 >= 0, r2_cut >= 0");
     hipStream_t st = (hipStream_t)stream;

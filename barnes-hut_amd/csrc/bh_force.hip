@@ -158,14 +158,50 @@ __device__ __forceinline__ float far_t32(float wx, float wy, float rM, float eps
 }
 
 // ---------------------------------------------------------------- FP64
-template <int P, int LAW>
+// Far-field weights M0 / q0 and M1 / q1 of two COM entries for one particle
+// from one reciprocal: Q = q0 q1, R = 1/Q from v_rcp_f64 refined as in
+// wdiv64, then M0 / q0 = (M0 R) q1 and M1 / q1 = (M1 R) q0.  That is the VALU
+// count of two separate refined weights with half the reciprocals (v_rcp_f64
+// takes four issue cycles of its own), at four roundings per weight instead
+// of two.  Needs every kept q >= 1, which the FP64 kernel's far entries have in
+// units of 1 / sig (their cell is at least one cell away); an entry the
+// particle does not keep (k0 / k1 false: inside its 3x3 block) enters the pair
+// as q = 1 and gets weight 0, so the other weight is the same as in any pair
+// with it.
+template <int LAW>
+__device__ __forceinline__ void pair_w64(double wx0, double wy0, double M0, bool k0, double wx1,
+                                         double wy1, double M1, bool k1, double eps2,
+                                         double r2cut, double& w0, double& w1)
+{
+    const double s0 = fma(wx0, wx0, wy0 * wy0), s1 = fma(wx1, wx1, wy1 * wy1);
+    double q0 = (LAW == 0) ? fma(wx0, wx0, fma(wy0, wy0, eps2)) : s0 + eps2;
+    double q1 = (LAW == 0) ? fma(wx1, wx1, fma(wy1, wy1, eps2)) : s1 + eps2;
+    q0 = k0 ? q0 : 1.0;
+    q1 = k1 ? q1 : 1.0;
+    const double Q = q0 * q1;
+    const double R0 = __builtin_amdgcn_rcp(Q);
+    const double e = fma(-Q, R0, 1.0);
+    const double R = fma(R0, fma(e, e, e), R0);
+    w0 = (M0 * R) * q1;
+    w1 = (M1 * R) * q0;
+    if (LAW == 1) {
+        w0 = s0 > r2cut ? w0 : 0.0;
+        w1 = s1 > r2cut ? w1 : 0.0;
+    }
+    w0 = k0 ? w0 : 0.0;
+    w1 = k1 ? w1 : 0.0;
+}
+
+// FAST 1: far-field weights in pairs (pair_w64), near-field weights by
+// wdiv64, positions in units of a power of two near the cell size; FAST 0:
+// IEEE division per weight, physical units (fp64_ieee).
+template <int P, int LAW, int FAST>
 __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
 {
     __shared__ Com64 ftile[BH_WAVE];
     __shared__ NearE ntile[BH_WAVE];
     const int lane = threadIdx.x;
     const int nitems = A.stats[ST_NITEMS];
-    const double eps2 = A.eps2, r2cut = A.r2cut;
     for (;;) {
         const int q = next_item(A.stats, lane);
         if (q >= nitems) break;
@@ -178,6 +214,18 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
         const int64_t coff = d.c_off;
         const int cx = it.cx;
         const int ylo = max(it.y0 - 1, 0), yhi = min(it.y1 + 1, G - 1);
+        // FAST: positions in units of 1 / sig, sig a power of two with
+        // step * sig in [1, 2).  The scaling is exact, so every difference and
+        // product rounds as in physical units, and a far-field q is >= 1.
+        double sig = 1.0;
+        if (FAST && d.step > 0.0 && d.step < __builtin_huge_val()) {
+            int e2;
+            (void)frexp(d.step, &e2);
+            sig = ldexp(1.0, 1 - e2);
+        }
+        const double eps2 = A.eps2 * sig * sig, r2cut = A.r2cut * sig * sig;
+        // past the list: zero mass far away (the term is exactly 0)
+        const double pad = FAST ? 1e30 : 1e150;
 
         double xi[P], yi[P], sx[P], sy[P];
         int cy[P], gi[P];
@@ -193,8 +241,8 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
             sx[k] = 0.0;
             sy[k] = 0.0;
             if (act[k]) {
-                xi[k] = A.x[gi[k]];
-                yi[k] = A.y[gi[k]];
+                xi[k] = A.x[gi[k]] * sig;
+                yi[k] = A.y[gi[k]] * sig;
                 cy[k] = (int)(A.key_sorted[gi[k]] - coff) - cx * G;
             }
         }
@@ -209,8 +257,8 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
                 const int j = base + lane;
                 NearE e;
                 if (j < pb) {
-                    e.x = A.x[j];
-                    e.y = A.y[j];
+                    e.x = A.x[j] * sig;
+                    e.y = A.y[j] * sig;
                     e.m = A.m[j];
                     e.row = (int)(A.key_sorted[j] - cl);
                     e.idx = j;
@@ -230,7 +278,7 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
                     for (int k = 0; k < P; ++k) {
                         const bool keep = (f.idx != gi[k]) && (abs(f.row - cy[k]) <= 1);
                         const double wx = f.x - xi[k], wy = f.y - yi[k];
-                        double w = near_w64<LAW>(wx, wy, f.m, eps2, r2cut);
+                        double w = near_w64<LAW, FAST>(wx, wy, f.m, eps2, r2cut);
                         w = keep ? w : 0.0;
                         sx[k] = fma(wx, w, sx[k]);
                         sy[k] = fma(wy, w, sy[k]);
@@ -255,31 +303,76 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
                 mhi[t] = 0;
             }
         }
-        Com64 nxt;
-        {
-            const int e = ne0 + lane;
-            if (e < ne1) nxt = A.l64[e];
-            else { nxt.X = 1e150; nxt.Y = 1e150; nxt.M = 0.0; nxt.kx = -100000; nxt.ky = -100000; }
-        }
+        auto stage = [&](int e) {
+            Com64 c;
+            if (e < ne1) {
+                c = A.l64[e];
+                c.X *= sig;
+                c.Y *= sig;
+            } else {
+                c.X = pad; c.Y = pad; c.M = 0.0; c.kx = -100000; c.ky = -100000;
+            }
+            return c;
+        };
+        Com64 nxt = stage(ne0 + lane);
         for (int base = ne0; base < ne1; base += BH_WAVE) {
             ftile[lane] = nxt;
             __syncthreads();
-            {
-                const int e = base + BH_WAVE + lane;
-                if (e < ne1) nxt = A.l64[e];
-                else { nxt.X = 1e150; nxt.Y = 1e150; nxt.M = 0.0; nxt.kx = -100000; nxt.ky = -100000; }
-            }
+            nxt = stage(base + BH_WAVE + lane);
             const int b1 = base + BH_WAVE;
             const bool masked = meets(base, b1, mlo[0], mhi[0]) || meets(base, b1, mlo[1], mhi[1]) ||
                                 meets(base, b1, mlo[2], mhi[2]);
-            if (!masked) {
+            if (FAST) {
+                // entries in pairs (fixed by their index, so every particle
+                // gets the same pairs under any launch shape) sharing one
+                // reciprocal; masked tiles give an entry a particle does not
+                // keep q = 1 inside the pair, so the kept one rounds exactly
+                // as in an unmasked tile
+                if (!masked) {
+#pragma unroll 2
+                    for (int j = 0; j < BH_WAVE; j += 2) {
+                        const Com64 f0 = ftile[j], f1 = ftile[j + 1];
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            const double wx0 = f0.X - xi[k], wy0 = f0.Y - yi[k];
+                            const double wx1 = f1.X - xi[k], wy1 = f1.Y - yi[k];
+                            double w0, w1;
+                            pair_w64<LAW>(wx0, wy0, f0.M, true, wx1, wy1, f1.M, true, eps2, r2cut,
+                                          w0, w1);
+                            sx[k] = fma(wx0, w0, sx[k]);
+                            sy[k] = fma(wy0, w0, sy[k]);
+                            sx[k] = fma(wx1, w1, sx[k]);
+                            sy[k] = fma(wy1, w1, sy[k]);
+                        }
+                    }
+                } else {
+                    for (int j = 0; j < BH_WAVE; j += 2) {
+                        const Com64 f0 = ftile[j], f1 = ftile[j + 1];
+                        const bool cn0 = abs(f0.kx - cx) <= 1, cn1 = abs(f1.kx - cx) <= 1;
+#pragma unroll
+                        for (int k = 0; k < P; ++k) {
+                            const bool k0 = !(cn0 && abs(f0.ky - cy[k]) <= 1);
+                            const bool k1 = !(cn1 && abs(f1.ky - cy[k]) <= 1);
+                            const double wx0 = f0.X - xi[k], wy0 = f0.Y - yi[k];
+                            const double wx1 = f1.X - xi[k], wy1 = f1.Y - yi[k];
+                            double w0, w1;
+                            pair_w64<LAW>(wx0, wy0, f0.M, k0, wx1, wy1, f1.M, k1, eps2, r2cut, w0,
+                                          w1);
+                            sx[k] = fma(wx0, w0, sx[k]);
+                            sy[k] = fma(wy0, w0, sy[k]);
+                            sx[k] = fma(wx1, w1, sx[k]);
+                            sy[k] = fma(wy1, w1, sy[k]);
+                        }
+                    }
+                }
+            } else if (!masked) {
 #pragma unroll 4
                 for (int j = 0; j < BH_WAVE; ++j) {
                     const Com64 f = ftile[j];
 #pragma unroll
                     for (int k = 0; k < P; ++k) {
                         const double wx = f.X - xi[k], wy = f.Y - yi[k];
-                        const double w = far_w64<LAW>(wx, wy, f.M, eps2, r2cut);
+                        const double w = far_w64<LAW, 0>(wx, wy, f.M, eps2, r2cut);
                         sx[k] = fma(wx, w, sx[k]);
                         sy[k] = fma(wy, w, sy[k]);
                     }
@@ -292,7 +385,7 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
                     for (int k = 0; k < P; ++k) {
                         const bool keep = !(colnear && abs(f.ky - cy[k]) <= 1);
                         const double wx = f.X - xi[k], wy = f.Y - yi[k];
-                        double w = far_w64<LAW>(wx, wy, f.M, eps2, r2cut);
+                        double w = far_w64<LAW, 0>(wx, wy, f.M, eps2, r2cut);
                         w = keep ? w : 0.0;
                         sx[k] = fma(wx, w, sx[k]);
                         sy[k] = fma(wy, w, sy[k]);
@@ -305,8 +398,8 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
 #pragma unroll
         for (int k = 0; k < P; ++k) {
             if (act[k]) {
-                A.ax[gi[k]] += A.Gc * sx[k];
-                A.ay[gi[k]] += A.Gc * sy[k];
+                A.ax[gi[k]] += A.Gc * (sx[k] * sig);
+                A.ay[gi[k]] += A.Gc * (sy[k] * sig);
             }
         }
         if (A.cost_log && lane == 0) A.cost_log[itx] = (long long)wall_clock64() - t0;
@@ -911,21 +1004,24 @@ __global__ __launch_bounds__(64) void k_force32m(ForceArgs A)
 // ---------------------------------------------------------------- dispatch table
 typedef void (*force_fn)(ForceArgs);
 
-#define F64(P, L) (force_fn)k_force64<P, L>
+#define F64(P, L) (force_fn)k_force64<P, L, 1>
+#define F64I(P, L) (force_fn)k_force64<P, L, 0>
 #define F32(P, L, K) (force_fn)k_force32<P, L, K>
 #define F32M(P, L) (force_fn)k_force32m<P, L>
 
-// [precision: 0 fp64, 1..3 fp32 kahan mode 1,2,0, 4 fp32 matrix-core][P index][law]
-static const force_fn g_force[5][3][2] = {
+// [precision: 0 fp64, 1..3 fp32 kahan mode 1,2,0, 4 fp32 matrix-core, 5 fp64 IEEE
+// division][P index][law]
+static const force_fn g_force[6][3][2] = {
     {{F64(1, 0), F64(1, 1)}, {F64(2, 0), F64(2, 1)}, {F64(4, 0), F64(4, 1)}},
     {{F32(1, 0, 1), F32(1, 1, 1)}, {F32(2, 0, 1), F32(2, 1, 1)}, {F32(4, 0, 1), F32(4, 1, 1)}},
     {{F32(1, 0, 2), F32(1, 1, 2)}, {F32(2, 0, 2), F32(2, 1, 2)}, {F32(4, 0, 2), F32(4, 1, 2)}},
     {{F32(1, 0, 0), F32(1, 1, 0)}, {F32(2, 0, 0), F32(2, 1, 0)}, {F32(4, 0, 0), F32(4, 1, 0)}},
     {{F32M(1, 0), F32M(1, 1)}, {F32M(2, 0), F32M(2, 1)}, {F32M(4, 0), F32M(4, 1)}},
+    {{F64I(1, 0), F64I(1, 1)}, {F64I(2, 0), F64I(2, 1)}, {F64I(4, 0), F64I(4, 1)}},
 };
 
 const void* bh_force_kernel(int precision, int pidx, int law)
 {
-    if (precision < 0 || precision > 4 || pidx < 0 || pidx > 2 || law < 0 || law > 1) return nullptr;
+    if (precision < 0 || precision > 5 || pidx < 0 || pidx > 2 || law < 0 || law > 1) return nullptr;
     return (const void*)g_force[precision][pidx][law];
 }

@@ -44,23 +44,44 @@ struct ForceArgs {
     int rc;
 };
 
-// One FP64 weight m * w(r2) of the law (LAW 0: no cutoff; LAW 1: r2 <= r2cut
-// gives 0): the expressions of the FP64 force kernel, shared with the FP64
-// recompute so that both produce the same terms.
-template <int LAW>
-__device__ __forceinline__ double far_w64(double wx, double wy, double M, double eps2, double r2cut)
+// m / q to FP64 accuracy without the IEEE division sequence: v_rcp_f64 gives
+// r0 with e = 1 - q r0 small (|e| < 2^-22), and 1/q = r0 (1 + e + e^2 + e^3 + ...),
+// so m r0 (1 + e + e^2) differs from m / q by the relative e^3 plus the
+// roundings of m r0 and of the final fma: within about one ulp.  Three VALU
+// instructions and the reciprocal, against ten and the reciprocal for the
+// correctly rounded quotient.  q = 0 gives NaN (m / 0 * 0 is NaN as well).
+__device__ __forceinline__ double wdiv64(double m, double q)
 {
-    if (LAW == 0) return M / fma(wx, wx, fma(wy, wy, eps2));
-    const double s2 = fma(wx, wx, wy * wy);
-    return (s2 > r2cut) ? M / (s2 + eps2) : 0.0;
+    const double r0 = __builtin_amdgcn_rcp(q);
+    const double e = fma(-q, r0, 1.0);
+    const double ee = fma(e, e, e);
+    const double w0 = m * r0;
+    return fma(w0, ee, w0);
 }
 
-template <int LAW>
+// One FP64 weight m * w(r2) of the law (LAW 0: no cutoff; LAW 1: r2 <= r2cut
+// gives 0).  FAST 1: the quotient by wdiv64 (the FP64 kernel's near field);
+// FAST 0: the IEEE division (the fp64_ieee kernel).
+template <int FAST>
+__device__ __forceinline__ double qdiv64(double m, double q)
+{
+    return FAST ? wdiv64(m, q) : m / q;
+}
+
+template <int LAW, int FAST>
+__device__ __forceinline__ double far_w64(double wx, double wy, double M, double eps2, double r2cut)
+{
+    if (LAW == 0) return qdiv64<FAST>(M, fma(wx, wx, fma(wy, wy, eps2)));
+    const double s2 = fma(wx, wx, wy * wy);
+    return (s2 > r2cut) ? qdiv64<FAST>(M, s2 + eps2) : 0.0;
+}
+
+template <int LAW, int FAST>
 __device__ __forceinline__ double near_w64(double wx, double wy, double m, double eps2, double r2cut)
 {
     const double s2 = fma(wx, wx, wy * wy);
-    if (LAW == 0) return m / (s2 + eps2);
-    return (s2 > r2cut) ? m / (s2 + eps2) : 0.0;
+    if (LAW == 0) return qdiv64<FAST>(m, s2 + eps2);
+    return (s2 > r2cut) ? qdiv64<FAST>(m, s2 + eps2) : 0.0;
 }
 
 // placement

@@ -380,22 +380,12 @@ __global__ __launch_bounds__(256) void k_rc_groups(const int32_t* __restrict__ r
     }
 }
 
-// 1 / q to FP64 accuracy: v_rcp_f64 refined by two Newton steps.
-__device__ __forceinline__ double rcp64(double q)
-{
-    double r = __builtin_amdgcn_rcp(q);
-    double e = fma(-q, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-q, r, 1.0);
-    return fma(r, e, r);
-}
-
 template <int LAW>
 __device__ __forceinline__ double rc_w64(double wx, double wy, double m, double eps2, double r2cut)
 {
     const double s2 = fma(wx, wx, wy * wy);
-    if (LAW == 0) return m * rcp64(s2 + eps2);
-    return (s2 > r2cut) ? m * rcp64(s2 + eps2) : 0.0;
+    if (LAW == 0) return wdiv64(m, s2 + eps2);
+    return (s2 > r2cut) ? wdiv64(m, s2 + eps2) : 0.0;
 }
 
 __device__ __forceinline__ int rc_slices(int ngroups)

@@ -1,14 +1,28 @@
 #!/usr/bin/env python3
 """Throughput of the grid-method step on one MI355X.
 
-Workload: N = 4,194,304 particles in 2-D, a mixture of 64 isotropic Gaussians
-(centres U[0,1)^2, sigma log-uniform [0.005, 0.05], clipped to the unit box),
-masses U[1,2], zero initial velocity, a 1024 x 1024 grid; law of the
-reference's GPU kernel (no softening, no cutoff), G = 1e-6, dt = 1e-3.
+Workload (`value`): N = 4,194,304 particles in 2-D, a mixture of 64 isotropic
+Gaussians (centres U[0,1)^2, sigma log-uniform [0.005, 0.05], clipped to the
+unit box), masses U[1,2], zero initial velocity, a 1024 x 1024 grid; the reference's
+law a_i = G sum m_j (x_j - x_i) / (r^2 + eps^2) with softening eps = 1e-5 (a
+third of the mean particle spacing in the densest cluster), no cutoff,
+G = 1e-6, dt = 1e-5.
+
+The timed window is always steps 1..K of that initial distribution: the
+warm-up steps run first, then the initial state is loaded again.  The run is
+resolved: the softening caps any pair's acceleration at G m / (2 eps) = 0.1,
+whose orbital time (~1e-2) is a thousand steps, and over 25 steps (2.5e-4
+time units, < 1 % of the clusters' free-fall time of ~0.03) particles move
+~1e-7 of the box, so every step places the same distribution and does the
+same interactions (1.99e12 per step), whatever --steps and --warmup are.
+Softening costs nothing: the kernels add eps^2 to r^2 either way.
 
 A "step" is one full round: bounding box + placement + ordered permutation,
 per-cell COM, forces, tick.  Paths on the same input:
-  * plain FP64, static work-queue order -- the rebuilt reference path;
+  * plain FP64, static work-queue order -- the rebuilt reference path, each
+    weight m / r^2 from the hardware reciprocal plus one third-order
+    refinement (within about one ulp of the quotient);
+  * the same with the IEEE division per weight (`fp64_ieee`, secondary);
   * FP32 cell-relative forces, Kahan-compensated tile sums, static order;
   * the same FP32 path with the learned work-queue order (the cost model in
     barnes-hut_amd/costmodels/clusters256.npz; --cost-model none disables).
@@ -17,10 +31,13 @@ nearly cancel (kappa 2, --recompute-kappa 0 disables); it is timed apart from
 the force kernel.  `value` is the FP32 + learned-order path's whole-job
 throughput.
 
-Secondary lines (one untimed-in-`value` measurement each): 512 independent
-systems stepped by one graph launch per step (static vs learned order), and
-the reference's own logged configuration (N = 10M uniform in [0,499)^2, 120 x
-120 grid, FP64, G = 1, dt = 0.005) against its published 166.837 s/step.
+Secondary blocks (not in `value`): the cost-model workload (256 clusters,
+sigma [0.001, 0.03], lognormal masses, 1024^2: FP64, FP32 static, FP32
+learned), 512 independent systems stepped by one graph launch per step, the
+reference's CPU-runnable case (16384 uniform particles, 32^2, eps = 1e-3: the
+oracle's full step on the host cores and the GPU step), and the reference's
+own logged configuration (N = 10M uniform in [0,499)^2, 120 x 120 grid, FP64,
+G = 1, dt = 0.005) against its published 166.837 s/step.
 
 Prints one JSON line.
 """
@@ -28,7 +45,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -46,11 +62,14 @@ N_DEFAULT = 4194304
 GRID_DEFAULT = 1024
 N_CLUSTERS = 64
 SIGMA = (0.005, 0.05)
-GC, DT, EPS2, R2CUT = 1e-6, 1e-3, 0.0, 0.0
+SEED = 2024
+GC, DT, EPS2, R2CUT = 1e-6, 1e-5, 1e-10, 0.0
+# cost-model workload: 256 clusters, sigma log-uniform [0.001, 0.03], lognormal masses
+C4_CLUSTERS, C4_SIGMA, C4_SEED = 256, (0.001, 0.03), 91
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_INTERACTION = 11
-# HBM bytes per force-kernel launch on this workload: rocprofv3 --pmc
+# HBM bytes per FP32 force-kernel launch on this workload: rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (KiB per dispatch), scaled by the readings of a
 # 1 GiB device copy (FETCH x2.0, WRITE x1.0), measured on the first two
 # steps of this workload with the default launch shape
@@ -59,7 +78,11 @@ TRAFFIC = 5.10e9
 # Issue-limited ceiling of the FP32 far-field term: 6 VALU instructions per
 # interaction (v_rcp_f32 co-issued), no packed FP32, 256 CUs x 4 SIMDs x 16
 # lanes x 2.4 GHz = 39.3e12 lane-instructions/s -> 6.55e12 interactions/s.
-ISSUE_CEILING_TFLOPS = 256 * 4 * 16 * 2.4e9 / 6 * FLOP_PER_INTERACTION / 1e12
+LANE_OPS = 256 * 4 * 16 * 2.4e9
+ISSUE_CEILING_TFLOPS = LANE_OPS / 6 * FLOP_PER_INTERACTION / 1e12
+# FP64 far-field term: 10 VALU instructions (2 differences, 2 fma for r^2, 3
+# for the refined quotient, m * r0, 2 fma into the sums) + v_rcp_f64
+ISSUE_CEILING64_TFLOPS = LANE_OPS / 10 * FLOP_PER_INTERACTION / 1e12
 # FP64 recompute threshold of the FP32 paths (--recompute-kappa)
 RECOMPUTE_KAPPA = 2.0
 
@@ -68,14 +91,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_path(eng, precision, steps, warmup, order_fn=None, kappa=RECOMPUTE_KAPPA):
-    """Warm-up, then `steps` timed full steps.  Returns (seconds per step,
-    force-kernel ms per step, recompute ms per step, near and far interactions
-    per step).  The FP32 force pass runs deferred, so that HIP events split the
-    force kernels (with the slice combine) from the FP64 recompute stage."""
+def run_path(eng, d, precision, steps, warmup, order_fn=None, kappa=RECOMPUTE_KAPPA):
+    """Warm-up from the initial state `d`, then `d` again and `steps` timed
+    full steps.  Returns (seconds per step, force-kernel ms per step,
+    recompute ms per step, near and far interactions per step).  The FP32
+    force pass runs deferred, so that HIP events split the force kernels (with
+    the slice combine) from the FP64 recompute stage."""
     import torch
     ctx = eng.ctx
-    fp32 = precision != "fp64"
+    fp32 = not precision.startswith("fp64")
     if fp32:
         ctx.set_recompute(kappa, deferred=True)
 
@@ -103,6 +127,7 @@ def run_path(eng, precision, steps, warmup, order_fn=None, kappa=RECOMPUTE_KAPPA
     for _ in range(warmup):
         one()
     eng.check()
+    eng.load(d)   # the timed window starts from the initial state
     torch.cuda.synchronize()
     evs = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(steps)]
     t0 = time.perf_counter()
@@ -119,13 +144,58 @@ def run_path(eng, precision, steps, warmup, order_fn=None, kappa=RECOMPUTE_KAPPA
     return (t1 - t0) / steps, kms, rms, near, far
 
 
-def force_errors(d, G, eng_cls, kappa=RECOMPUTE_KAPPA):
-    """FP32 vs FP64 accelerations on the same placement: normwise, p99 and
-    max per-particle relative error over all N particles (FP64 GPU path as
-    the reference), with and without the FP64 recompute.  The FP64 path itself
-    is checked against the oracle by the tests, on this workload at full size."""
+def measure(d, G, precision, tuning, steps, warmup, model=None, kappa=RECOMPUTE_KAPPA,
+            autotune_first=False):
+    """One path on workload `d` (G x G grid): see run_path."""
     import torch
-    eng = eng_cls([len(d["px"])], [G])
+    from barneshut.engine import Engine
+    n = len(d["px"])
+    eng = Engine([n], [G], tuning=tuning)
+    eng.load(d)
+    if autotune_first:
+        from barneshut import autotune
+        eng.create_tree()
+        eng.summarize()
+        tuning = autotune.tune(eng, (GC, EPS2, R2CUT), precision, log=log)["tuning"]
+        eng.load(d)
+    order_fn = None
+    if model is not None:
+        order_buf = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
+
+        def order_fn():
+            eng.ctx.learned_order(model.params, order_buf)
+            return order_buf
+    sec, kms, rms, near, far = run_path(eng, d, precision, steps, warmup, order_fn, kappa)
+    st = eng.ctx.stats()
+    n_items, n_rc = int(st[2]), int(st[6])
+    eng.close()
+    flops = FLOP_PER_INTERACTION * (near + far).astype(np.float64)
+    return dict(seconds_per_step=sec, value=n / sec / 1e6, kms=kms,
+                force_kernel_ms=float(kms.mean()), recompute_ms=float(rms.mean()),
+                recomputed_last_step=n_rc, near=near, far=far,
+                flop_per_launch=float(flops.mean()),
+                achieved=float(flops.sum() / (kms.sum() * 1e-3) / 1e12), tuning=tuning,
+                work_items=n_items, steps=steps)
+
+
+def path_summary(r, peak):
+    return {"value": r["value"], "ms_per_step": r["seconds_per_step"] * 1e3,
+            "steps": r["steps"], "tuning": r["tuning"], "force_kernel_ms": r["force_kernel_ms"],
+            "recompute_ms": r["recompute_ms"], "achieved_tflops": r["achieved"],
+            "frac_of_peak": r["achieved"] / peak,
+            "interactions_first_last": [float(r["near"][0] + r["far"][0]),
+                                        float(r["near"][-1] + r["far"][-1])]}
+
+
+def force_errors(d, G, runs, kappa=RECOMPUTE_KAPPA):
+    """Accelerations of the initial state (the first timed step's): FP32
+    precisions and the IEEE-division FP64 kernel against the default FP64
+    kernel on the same placement, normwise, p99 and max per-particle relative
+    error over all N particles.  The FP64 path itself is checked against the
+    oracle by the tests, on these workloads at full size."""
+    import torch
+    from barneshut.engine import Engine
+    eng = Engine([len(d["px"])], [G])
     eng.load(d)
     eng.create_tree()
     eng.summarize()
@@ -133,10 +203,6 @@ def force_errors(d, G, eng_cls, kappa=RECOMPUTE_KAPPA):
     eng.ctx.force_pass(eng.parts, GC, EPS2, R2CUT, "fp64")
     a64 = (eng.parts["ax"].clone(), eng.parts["ay"].clone())
     out = {}
-    runs = [("fp32", "fp32", kappa), ("fp32_no_recompute", "fp32", 0.0),
-            ("fp32_kahan_term", "fp32_kahan_term", kappa),
-            ("fp32_kahan_term_no_recompute", "fp32_kahan_term", 0.0),
-            ("fp32_plain_no_recompute", "fp32_plain", 0.0)]
     for name, prec, kap in runs:
         eng.ctx.set_recompute(kap)
         eng.parts["ax"].zero_()
@@ -148,19 +214,27 @@ def force_errors(d, G, eng_cls, kappa=RECOMPUTE_KAPPA):
         out[name] = dict(normwise=float(torch.sqrt((num ** 2).sum() / (den ** 2).sum())),
                          p99=float(np.quantile(per, 0.99)), max=float(per.max()),
                          n_over_1e5=int((per > 1e-5).sum()))
-        if kap > 0:
+        if kap > 0 and not prec.startswith("fp64"):
             out[name]["recomputed"] = int(eng.ctx.stats()[6])
     eng.check()
     eng.close()
     return out
 
 
-def cpu_baseline(d, G, eng_cls=None, budget_s=12.0):
+C2_ERROR_RUNS = [("fp32", "fp32", RECOMPUTE_KAPPA), ("fp32_no_recompute", "fp32", 0.0),
+                 ("fp32_kahan_term", "fp32_kahan_term", RECOMPUTE_KAPPA),
+                 ("fp32_kahan_term_no_recompute", "fp32_kahan_term", 0.0),
+                 ("fp32_plain_no_recompute", "fp32_plain", 0.0),
+                 ("fp64_ieee", "fp64_ieee", 0.0)]
+
+
+def cpu_baseline(d, G, budget_s=12.0):
     """The C restatement (oracle/) of python-bh's sequential step on this host:
     full placement, COM and tick over all N; forces on a random sample of
-    target particles (OpenMP over targets), extrapolated to N.  With eng_cls,
-    the GPU placement of the same state is then checked against the timed
-    oracle placement (bounding box, cell keys, counts, offsets, permutation)."""
+    target particles (OpenMP over targets), extrapolated to N.  The GPU
+    placement of the same state is then checked against the timed oracle
+    placement (bounding box, cell keys, counts, offsets, permutation)."""
+    from barneshut.engine import Engine
     from oracle import oracle as orc
     cores = len(os.sched_getaffinity(0))
     n = len(d["px"])
@@ -193,29 +267,81 @@ def cpu_baseline(d, G, eng_cls=None, budget_s=12.0):
     orc.tick(xx, yy, vx, vy, ax, ay, DT)
     t_tick = time.perf_counter() - tb
     t_step = (t1 - t0) + (t2 - t1) + t_force + t_tick
-    check = None
-    if eng_cls is not None:   # after the timing: the oracle's placement as the checker
-        cnt, _ = orc.counts_offsets(ks, G * G)
-        eng = eng_cls([n], [G])
-        eng.load(d)
-        eng.create_tree()
-        eng.check()
-        gbb = eng.bbox()[0]
-        check = {"n": n,
-                 "bbox": bool(gbb[0] == bb[0] and gbb[1] == bb[1] and gbb[2] == bb[2]
-                              and gbb[4] == bb[4]),
-                 "keys": bool(np.array_equal(eng.keys_in(), key)),
-                 "counts": bool(np.array_equal(eng.counts(), cnt)),
-                 "offsets": bool(np.array_equal(eng.offsets(), off)),
-                 "permutation": bool(np.array_equal(eng.perm(), perm))}
-        eng.close()
+    # after the timing: the oracle's placement as the checker
+    cnt, _ = orc.counts_offsets(ks, G * G)
+    eng = Engine([n], [G])
+    eng.load(d)
+    eng.create_tree()
+    eng.check()
+    gbb = eng.bbox()[0]
+    check = {"n": n,
+             "bbox": bool(gbb[0] == bb[0] and gbb[1] == bb[1] and gbb[2] == bb[2]
+                          and gbb[4] == bb[4]),
+             "keys": bool(np.array_equal(eng.keys_in(), key)),
+             "counts": bool(np.array_equal(eng.counts(), cnt)),
+             "offsets": bool(np.array_equal(eng.offsets(), off)),
+             "permutation": bool(np.array_equal(eng.perm(), perm))}
+    eng.close()
     return {"value": n / t_step / 1e6, "unit": "Mparticle-steps/s", "cores": cores,
             "kind": "port",
-            "sample": (f"C restatement of the sequential step (oracle/bh_oracle.c): placement "
-                       f"({t1 - t0:.2f}s, 1 thread), COM ({t2 - t1:.2f}s) and tick over all "
-                       f"{n} particles; forces on {ns} random particles with {cores} OpenMP "
-                       f"threads, x{n / ns:.1f} extrapolated ({t_force:.1f}s/step)"),
+            "sample": (f"C restatement of the sequential step (oracle/bh_oracle.c) on the initial "
+                       f"state (the first timed step's): placement ({t1 - t0:.2f}s, 1 thread), "
+                       f"COM ({t2 - t1:.2f}s) and tick over all {n} particles; forces on {ns} "
+                       f"random particles with {cores} OpenMP threads, x{n / ns:.1f} "
+                       f"extrapolated ({t_force:.1f}s/step)"),
             "seconds_per_step": t_step, "placement_check": check}
+
+
+def c1_line(steps):
+    """The reference's CPU-runnable case: 16384 particles U[0,1)^2, masses
+    U[1,2], zero velocity, G = 1, eps = 1e-3 (eps2 = 1e-6), 32 x 32 grid,
+    dt = 1e-4, one step.  The oracle's full step (placement, COM, forces of
+    every particle, tick) on the host cores, the GPU FP64 step, and the grid
+    method's own error against the FP64 direct sum (both sides)."""
+    from barneshut import inputs
+    from barneshut.engine import Engine
+    from oracle import oracle as orc
+    n, G, gc, eps2, dt = 16384, 32, 1.0, 1e-6, 1e-4
+    d = inputs.uniform(n, seed=1)
+    cores = len(os.sched_getaffinity(0))
+    t = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        ref = orc.step(d["px"], d["py"], d["mass"], d["vx"], d["vy"], d["pid"], G, gc, dt, eps2,
+                       0.0, nthreads=cores)
+        t.append(time.perf_counter() - t0)
+    t_cpu = min(t)
+    t1 = time.perf_counter()
+    ref1 = orc.step(d["px"], d["py"], d["mass"], d["vx"], d["vy"], d["pid"], G, gc, dt, eps2,
+                    0.0, nthreads=1)
+    t_cpu1 = time.perf_counter() - t1
+    ex, ey = orc.direct(ref["px"], ref["py"], ref["m"], gc, eps2, 0.0, nthreads=cores)
+    e_orc = float(np.sqrt(((ref["ax"] - ex) ** 2 + (ref["ay"] - ey) ** 2).sum()
+                          / (ex ** 2 + ey ** 2).sum()))
+    import torch
+    eng = Engine([n], [G], tuning=dict(p=1, yspan=512, waves_per_cu=32, tail_split=1,
+                                       tail_items=0))
+    eng.load(d)
+    eng.step(gc, eps2, 0.0, "fp64", dt, do_tick=False)
+    eng.check()
+    g = eng.fetch()
+    e_gpu = float(np.sqrt(((g["ax"] - ex) ** 2 + (g["ay"] - ey) ** 2).sum()
+                          / (ex ** 2 + ey ** 2).sum()))
+    eng.load(d)
+    eng.step(gc, eps2, 0.0, "fp64", dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.step(gc, eps2, 0.0, "fp64", dt)
+    torch.cuda.synchronize()
+    t_gpu = (time.perf_counter() - t0) / steps
+    eng.check()
+    eng.close()
+    return {"n": n, "grid": G, "G": gc, "eps2": eps2, "dt": dt,
+            "cpu_full_step_s": t_cpu, "cpu_value": n / t_cpu / 1e6, "cpu_cores": cores,
+            "cpu_full_step_1thread_s": t_cpu1, "cpu_kind": "port (oracle/bh_oracle.c)",
+            "gpu_fp64_ms_per_step": t_gpu * 1e3, "gpu_value": n / t_gpu / 1e6,
+            "method_error_vs_direct_sum": {"oracle": e_orc, "gpu_fp64": e_gpu}}
 
 
 # launch shapes the autotuner tries for the batched line (untimed, cached)
@@ -252,7 +378,7 @@ def batched_line(steps, model_path, seed=0):
     return out
 
 
-def reference_config_line(steps):
+def reference_config_line(steps, tuning64):
     """The reference's logged run (python-bh/data/frontera/nbody_out_singlegpu.dat):
     N = 10,000,000 uniform in [0,499)^2 (bin/gen_input.py), 120 x 120 grid,
     G = 1, dt = 0.005, FP64, one step per run; 166.837 s/step published."""
@@ -260,24 +386,30 @@ def reference_config_line(steps):
     from barneshut.engine import Engine
     n, G = 10_000_000, 120
     d = inputs.reference_normal(n, seed=1)
-    e = Engine([n], [G], tuning=dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0))
-    e.load(d)
+    out = {"n": n, "grid": G}
     import torch
-    e.step(1.0, 0.0, 0.0, "fp64", 0.005)
-    e.check()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        e.step(1.0, 0.0, 0.0, "fp64", 0.005)
-    torch.cuda.synchronize()
-    sec = (time.perf_counter() - t0) / steps
-    e.check()
-    e.close()
+    for prec in ("fp64", "fp64_ieee"):
+        e = Engine([n], [G], tuning=tuning64)
+        e.load(d)
+        e.step(1.0, 0.0, 0.0, prec, 0.005)
+        e.check()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            e.step(1.0, 0.0, 0.0, prec, 0.005)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        e.check()
+        e.close()
+        out[prec] = {"ms_per_step": sec * 1e3, "value": n / sec / 1e6}
     published = 166.83713   # s/step, nbody_out_singlegpu.dat:35
-    return {"n": n, "grid": G, "precision": "fp64", "ms_per_step": sec * 1e3,
-            "value": n / sec / 1e6, "published_s_per_step": published,
-            "published_value": n / published / 1e6, "speedup_vs_published": published / sec,
-            "published_hardware": "one GPU of the reference's Frontera runs (model not recorded)"}
+    sec = out["fp64"]["ms_per_step"] * 1e-3
+    out.update({"precision": "fp64", "ms_per_step": sec * 1e3, "value": n / sec / 1e6,
+                "published_s_per_step": published, "published_value": n / published / 1e6,
+                "speedup_vs_published": published / sec,
+                "published_hardware": "one GPU of the reference's Frontera runs (model not "
+                                      "recorded)"})
+    return out
 
 
 def main():
@@ -288,7 +420,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=N_DEFAULT)
     ap.add_argument("--grid", type=int, default=GRID_DEFAULT)
-    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--seed", type=int, default=SEED)
     ap.add_argument("--p", type=int, default=2, help="FP32 path: particles per lane (1, 2, 4)")
     ap.add_argument("--yspan", type=int, default=512, help="max rows per work item")
     ap.add_argument("--waves-per-cu", type=int, default=32)
@@ -296,18 +428,24 @@ def main():
                     help="FP32 path: far-field slices per split item (1 = off)")
     ap.add_argument("--tail-items", type=int, default=1 << 20,
                     help="FP32 path: split the last this many queue items (>= items: all)")
-    ap.add_argument("--p64", type=int, default=1, help="FP64 path: particles per lane")
+    ap.add_argument("--p64", type=int, default=2, help="FP64 path: particles per lane")
+    ap.add_argument("--yspan64", type=int, default=512, help="FP64 path: max rows per item")
+    ap.add_argument("--waves64", type=int, default=32, help="FP64 path: waves per CU")
+    ap.add_argument("--ieee-steps", type=int, default=3,
+                    help="timed steps of the IEEE-division FP64 line (0 = skip)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick each path's launch shape with barneshut.autotune first")
     ap.add_argument("--cost-model", default=os.path.join(PKG, "costmodels", "clusters256.npz"),
                     help="trained cost-model file (.npz) or 'none'")
     ap.add_argument("--batched-model", default=os.path.join(PKG, "costmodels", "batched512.npz"))
+    ap.add_argument("--c4-steps", type=int, default=5,
+                    help="timed steps of each cost-model-workload path (0 = skip the block)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-errors", action="store_true")
     ap.add_argument("--skip-extra", action="store_true",
-                    help="skip the batched and reference-config lines")
+                    help="skip the batched, reference-config and CPU-case blocks")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
@@ -324,7 +462,6 @@ def main():
 
     import torch
     from barneshut import inputs
-    from barneshut.engine import Engine
 
     if args.calibrate_copy_mib:
         src = torch.ones(args.calibrate_copy_mib * (1 << 18), dtype=torch.float32, device="cuda")
@@ -338,62 +475,34 @@ def main():
     G = args.grid
     tuning = dict(p=args.p, yspan=args.yspan, waves_per_cu=args.waves_per_cu,
                   tail_split=args.tail_split, tail_items=args.tail_items)
-    tuning64 = dict(p=args.p64, yspan=args.yspan, waves_per_cu=args.waves_per_cu, tail_split=1,
+    tuning64 = dict(p=args.p64, yspan=args.yspan64, waves_per_cu=args.waves64, tail_split=1,
                     tail_items=0)
-
-    def measure(precision, tun, order_model=None):
-        eng = Engine([args.n], [G], tuning=tun)
-        eng.load(d)
-        if args.autotune:
-            from barneshut import autotune
-            eng.create_tree()
-            eng.summarize()
-            tun = autotune.tune(eng, (GC, EPS2, R2CUT), precision, log=log)["tuning"]
-            eng.load(d)
-        order_fn = None
-        if order_model is not None:
-            order_buf = torch.empty(eng.ctx.query(4), dtype=torch.int32, device=eng.dev)
-
-            def order_fn():
-                eng.ctx.learned_order(order_model.params, order_buf)
-                return order_buf
-        sec, kms, rms, near, far = run_path(eng, precision, args.steps, args.warmup, order_fn,
-                                            kappa)
-        st = eng.ctx.stats()
-        n_items, n_rc = int(st[2]), int(st[6])
-        eng.close()
-        flops = FLOP_PER_INTERACTION * (near + far).astype(np.float64)
-        return dict(seconds_per_step=sec, value=args.n / sec / 1e6, kms=kms,
-                    force_kernel_ms=float(kms.mean()), recompute_ms=float(rms.mean()),
-                    recomputed_last_step=n_rc, near=near, far=far,
-                    flop_per_launch=float(flops.mean()),
-                    achieved=float(flops.sum() / (kms.sum() * 1e-3) / 1e12), tuning=tun,
-                    work_items=n_items)
-
-    res64 = None
-    if not args.skip_fp64:
-        log("fp64 plain path")
-        res64 = measure("fp64", tuning64)
-        log(f"fp64: {res64['seconds_per_step'] * 1e3:.1f} ms/step, "
-            f"force kernel {res64['force_kernel_ms']:.1f} ms")
-    res_static = None
-    if not args.skip_static:
-        log("fp32 path, static order")
-        res_static = measure("fp32", tuning)
-        log(f"fp32 static: {res_static['seconds_per_step'] * 1e3:.1f} ms/step, "
-            f"force kernel {res_static['force_kernel_ms']:.1f} ms")
+    tuning64_ieee = dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0)
     model = None
     if args.cost_model and args.cost_model != "none" and os.path.exists(args.cost_model):
         from barneshut import costmodel
         model = costmodel.CostModel.load(args.cost_model, device=torch.device("cuda"))
+
+    def run(name, dd, precision, tun, steps, warmup, mdl=None):
+        log(f"{name}: {precision}{' learned order' if mdl is not None else ''}")
+        r = measure(dd, G, precision, tun, steps, warmup, mdl, kappa, args.autotune)
+        log(f"{name}: {r['seconds_per_step'] * 1e3:.1f} ms/step, force kernel "
+            f"{r['force_kernel_ms']:.1f} ms, {r['achieved']:.1f} TFLOP/s")
+        return r
+
+    res64 = None if args.skip_fp64 else run("c2 fp64", d, "fp64", tuning64, args.steps,
+                                            args.warmup)
+    res64i = None
+    if not args.skip_fp64 and args.ieee_steps > 0:
+        res64i = run("c2 fp64_ieee", d, "fp64_ieee", tuning64_ieee, args.ieee_steps, 1)
+    res_static = None if args.skip_static else run("c2 fp32 static", d, "fp32", tuning,
+                                                   args.steps, args.warmup)
     if model is not None:
-        log("fp32 path, learned order")
-        res = measure("fp32", tuning, model)
+        res = run("c2 fp32 learned", d, "fp32", tuning, args.steps, args.warmup, model)
         sched = "learned"
-        log(f"fp32 learned: {res['seconds_per_step'] * 1e3:.1f} ms/step, "
-            f"force kernel {res['force_kernel_ms']:.1f} ms")
     else:
-        res = res_static if res_static is not None else measure("fp32", tuning)
+        res = res_static if res_static is not None else run("c2 fp32", d, "fp32", tuning,
+                                                            args.steps, args.warmup)
         sched = "static"
 
     out = {
@@ -410,7 +519,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": "2-D grid-method step, N=4,194,304, 64 Gaussian clusters, "
-                               "1024x1024 grid",
+                               "1024x1024 grid, timed steps 1..K of the initial state",
                    "n": args.n, "grid": G, "clusters": N_CLUSTERS, "sigma": list(SIGMA),
                    "mass": "U[1,2]", "G": GC, "dt": DT, "eps2": EPS2, "r2_cut": R2CUT,
                    "seed": args.seed, "schedule": sched, "cost_model":
@@ -418,7 +527,9 @@ def main():
                    "tuning": res["tuning"], "work_items": res["work_items"],
                    "recompute_kappa": kappa},
         "interactions_per_step": {"near": float(res["near"].mean()),
-                                  "far": float(res["far"].mean())},
+                                  "far": float(res["far"].mean()),
+                                  "first_step": float(res["near"][0] + res["far"][0]),
+                                  "last_step": float(res["near"][-1] + res["far"][-1])},
         "recompute": {"kappa": kappa, "ms_per_step": res["recompute_ms"],
                       "share_of_step": res["recompute_ms"] / (res["seconds_per_step"] * 1e3),
                       "particles_last_step": res["recomputed_last_step"]},
@@ -431,29 +542,51 @@ def main():
                      "frac_of_issue_ceiling": res["achieved"] / ISSUE_CEILING_TFLOPS},
     }
     if res_static is not None:
-        out["fp32_static"] = {"value": res_static["value"],
-                              "ms_per_step": res_static["seconds_per_step"] * 1e3,
-                              "force_kernel_ms": res_static["force_kernel_ms"],
-                              "recompute_ms": res_static["recompute_ms"],
-                              "achieved_tflops": res_static["achieved"]}
+        out["fp32_static"] = path_summary(res_static, FP32_PEAK_TFLOPS)
         out["learned_vs_static"] = res_static["seconds_per_step"] / res["seconds_per_step"]
     if res64 is not None:
-        out["fp64_plain"] = {"value": res64["value"], "ms_per_step": res64["seconds_per_step"] * 1e3,
-                             "tuning": res64["tuning"], "force_kernel_ms": res64["force_kernel_ms"],
-                             "achieved_tflops": res64["achieved"],
-                             "frac_of_fp64_peak": res64["achieved"] / FP64_PEAK_TFLOPS}
+        out["fp64_plain"] = path_summary(res64, FP64_PEAK_TFLOPS)
+        out["fp64_plain"]["kernel"] = "k_force64<P,0,1> (refined hardware reciprocal)"
+        out["fp64_plain"]["issue_ceiling"] = ISSUE_CEILING64_TFLOPS
+        out["fp64_plain"]["frac_of_issue_ceiling"] = res64["achieved"] / ISSUE_CEILING64_TFLOPS
         out["speedup_vs_fp64"] = res64["seconds_per_step"] / res["seconds_per_step"]
+    if res64i is not None:
+        out["fp64_ieee"] = path_summary(res64i, FP64_PEAK_TFLOPS)
+        out["fp64_ieee"]["kernel"] = "k_force64<P,0,0> (IEEE division)"
+        out["speedup_vs_fp64_ieee"] = res64i["seconds_per_step"] / res["seconds_per_step"]
     if not args.skip_errors:
         log("force errors")
-        out["force_rel_err"] = force_errors(d, G, Engine, kappa=kappa)
+        out["force_rel_err"] = force_errors(d, G, C2_ERROR_RUNS, kappa=kappa)
+    if args.c4_steps > 0:
+        log(f"cost-model workload: {C4_CLUSTERS} clusters, lognormal masses")
+        d4 = inputs.gaussian_mixture(args.n, C4_CLUSTERS, C4_SIGMA, seed=C4_SEED,
+                                     mass="lognormal")
+        c4 = {"workload": f"N={args.n}, {C4_CLUSTERS} Gaussian clusters, sigma "
+                          f"{list(C4_SIGMA)}, lognormal(0,0.5) masses, {G}x{G} grid",
+              "seed": C4_SEED, "steps": args.c4_steps}
+        r64 = run("c4 fp64", d4, "fp64", tuning64, args.c4_steps, 1)
+        rst = run("c4 fp32 static", d4, "fp32", tuning, args.c4_steps, 1)
+        c4["fp64"] = path_summary(r64, FP64_PEAK_TFLOPS)
+        c4["fp32_static"] = path_summary(rst, FP32_PEAK_TFLOPS)
+        c4["speedup_static_vs_fp64"] = r64["seconds_per_step"] / rst["seconds_per_step"]
+        if model is not None:
+            rle = run("c4 fp32 learned", d4, "fp32", tuning, args.c4_steps, 1, model)
+            c4["fp32_learned"] = path_summary(rle, FP32_PEAK_TFLOPS)
+            c4["speedup_learned_vs_fp64"] = r64["seconds_per_step"] / rle["seconds_per_step"]
+            c4["learned_vs_static"] = rst["seconds_per_step"] / rle["seconds_per_step"]
+        if not args.skip_errors:
+            c4["force_rel_err"] = force_errors(d4, G, C2_ERROR_RUNS[:2], kappa=kappa)
+        out["c4"] = c4
     if not args.skip_extra:
         log("batched systems")
         out["batched_512"] = batched_line(args.steps, args.batched_model)
         log("reference configuration")
-        out["reference_config"] = reference_config_line(3)
+        out["reference_config"] = reference_config_line(3, tuning64)
+        log("reference CPU-runnable case")
+        out["c1"] = c1_line(5)
     if not args.skip_cpu:
         log("cpu baseline")
-        cb = cpu_baseline(d, G, Engine)
+        cb = cpu_baseline(d, G)
         out["placement_bit_exact"] = cb.pop("placement_check")
         out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)

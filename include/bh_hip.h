@@ -41,12 +41,16 @@ enum {
 #define BH_DEVERR_ITEMS 4       /* work-item capacity exceeded */
 
 enum {
-    BH_PREC_FP64 = 0,            /* plain FP64 forces */
+    BH_PREC_FP64 = 0,            /* plain FP64 forces; each weight m / (r^2 + eps2) from the
+                                    hardware reciprocal with one third-order refinement
+                                    (within about one ulp of the quotient) */
     BH_PREC_FP32_KAHAN = 1,      /* FP32 cell-relative, Kahan-summed 64-term tiles */
     BH_PREC_FP32_KAHAN_TERM = 2, /* FP32 cell-relative, Kahan per term */
     BH_PREC_FP32_PLAIN = 3,      /* FP32 cell-relative, uncompensated */
-    BH_PREC_FP32_MFMA = 4        /* FP32 cell-relative, far-field sums on the matrix cores,
+    BH_PREC_FP32_MFMA = 4,       /* FP32 cell-relative, far-field sums on the matrix cores,
                                     Kahan-summed 64-term tiles */
+    BH_PREC_FP64_IEEE = 5        /* plain FP64 forces with the IEEE (correctly rounded)
+                                    division per weight */
 };
 
 /* SoA particle buffers (device), length = total particles of all systems. */

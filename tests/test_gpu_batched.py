@@ -96,8 +96,21 @@ def test_autotuner_picks_admissible_shape_and_caches(tmp_path):
     assert r["tuning"] == best["tuning"]
     assert e.ctx.get_tuning() == r["tuning"]
     print(f"[autotune] best {r['tuning']} {r['ms']:.3f} ms", flush=True)
-    hit = autotune.tune(e, law, "fp32", candidates=cand, cache=cache)
-    assert hit["tuning"] == r["tuning"] and "table" not in hit
+    hit = autotune.tune(e, law, "fp32", candidates=cand, max_rel_err=1e-5, cache=cache)
+    assert hit["tuning"] == r["tuning"] and "table" not in hit and hit["cached"]
+    assert hit["rel_err"] <= 1e-5      # admitted again on this state
+    # the key covers every system, the law and the threshold: a batch that
+    # differs only in its last system, another law or another threshold misses
+    import types
+    k0 = autotune._key(e, "fp32", law, 1e-5)
+    n2 = sizes.copy()
+    n2[-1] += 1
+    other = types.SimpleNamespace(n=n2, grid_dim=grids, dev=e.dev)
+    assert autotune._key(other, "fp32", law, 1e-5) != k0
+    assert autotune._key(e, "fp32", (1e-6, 0.0, 0.0), 1e-5) != k0
+    assert autotune._key(e, "fp32", law, 1e-6) != k0
+    assert autotune._key(types.SimpleNamespace(n=sizes.copy(), grid_dim=grids.copy(), dev=e.dev),
+                         "fp32", law, 1e-5) == k0
     with pytest.raises(RuntimeError):     # an impossible error bound admits nothing
         autotune.tune(e, law, "fp32", candidates=cand, max_rel_err=1e-15, use_cache=False,
                       cache=None)

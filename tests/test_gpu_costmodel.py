@@ -91,13 +91,15 @@ def test_order_by_cost_is_descending_permutation():
     e = _engine(d, 64, dict(p=1, yspan=8, waves_per_cu=8))
     n = int(e.ctx.stats()[hiplib.ST_NITEMS])
     rng = np.random.default_rng(6)
-    cost = rng.normal(size=n).astype(np.float32)
-    order = torch.full((e.ctx.query(4),), -1, dtype=torch.int32, device="cuda")
+    cap = e.ctx.query(4)   # buffers are sized for the work-item capacity
+    cost = np.zeros(cap, np.float32)
+    cost[:n] = rng.normal(size=n)
+    order = torch.full((cap,), -1, dtype=torch.int32, device="cuda")
     e.ctx.order_by_cost(_t(cost), order)
     o = order[:n].cpu().numpy()
     assert sorted(o.tolist()) == list(range(n))
     c = cost[o]
-    width = (cost.max() - cost.min()) / 4095
+    width = (cost[:n].max() - cost[:n].min()) / 4095
     assert (np.diff(c) <= width * 1.001).all()
     # a non-finite prediction falls back to the static order
     cost[n // 2] = np.nan

@@ -152,7 +152,37 @@ def test_invalid_arguments_raise():
         e.ctx.evaluate(e.parts, 1.0, -1.0, 0.0, "fp64")
     with pytest.raises(BhError):
         e.ctx.sort(e.parts, e.parts)
+    with pytest.raises(ValueError, match="elements"):   # a short particle buffer
+        e.ctx.place({k: v[:500] for k, v in e.parts.items()})
     e.close()
+
+
+def test_refused_reconfiguration_keeps_the_context_usable():
+    """bh_set_systems validates every system before it releases the old
+    buffers: a refused call (grid side beyond the cap, an empty system) leaves
+    the configured context as it was, and its next step equals the step of a
+    context that never saw the refused call."""
+    import torch
+    from barneshut.engine import Engine
+    from barneshut.kernels.hiplib import BhError
+    d = inputs.gaussian_mixture(3000, 4, (0.02, 0.1), seed=41)
+    out = []
+    for refuse in (False, True):
+        e = Engine([3000], [16])
+        e.load(d)
+        if refuse:
+            with pytest.raises(BhError, match="32767"):
+                e.ctx.set_systems([3000], [32768])
+            with pytest.raises(BhError, match="n < 1"):
+                e.ctx.set_systems([3000, 0], [16, 4])
+            assert e.ctx.query(0) == 3000 and e.ctx.query(1) == 256
+        e.step(1.0, 1e-6, 0.0, "fp64", 1e-4)
+        e.check()
+        torch.cuda.synchronize()
+        out.append(e.fetch())
+        e.close()
+    for k in ("px", "py", "vx", "vy", "pid"):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
 
 
 @pytest.mark.parametrize("n", [1, 64])

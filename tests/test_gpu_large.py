@@ -3,8 +3,8 @@ clusters plus a 20 % uniform background, 4096 x 4096 grid) and of the energy
 diagnostic used for its drift report.
 
 The full-size case costs ~1e15 far-field interactions per step (about three
-GPU-minutes per FP32 step), so it runs only with BH_LARGE=1 and checks one
-step through properties that do not depend on size: placement keys bit-exact
+GPU-minutes per FP32 step), so it checks one step through properties that do
+not depend on size: placement keys bit-exact
 against the oracle, rows sorted under the reference's total order and a
 permutation of the input, COMs against the oracle, FP32 accelerations of a
 random sample of particles against the oracle's grid-method sum.  The energy
@@ -26,7 +26,8 @@ pytestmark = [pytest.mark.gpu,
 from oracle import oracle as orc  # noqa: E402
 from barneshut import inputs  # noqa: E402
 
-GC, DT = 1e-6, 1e-3              # law and step of the 4M-particle workload
+GC, DT = 1e-6, 1e-3              # law and step of the energy-drift runs
+BENCH_DT, BENCH_EPS2 = 1e-5, 1e-10   # the bench's resolved step and softening
 SIGMA = (0.001, 0.03)
 MODEL = os.path.join(PKG, "costmodels", "clusters256.npz")
 
@@ -137,8 +138,6 @@ def _sorted_under_total_order(h, keys):
     return bool(ok.all())
 
 
-@pytest.mark.skipif(os.environ.get("BH_LARGE") != "1",
-                    reason="67M-particle case: ~3 GPU-minutes per step; set BH_LARGE=1")
 def test_large_full_size_one_step():
     import torch
     from barneshut import costmodel
@@ -202,7 +201,7 @@ def test_large_full_size_one_step():
     np.testing.assert_allclose(com[:, :2], ref[:, :2], rtol=1e-12, atol=1e-15)
 
     # FP32 accelerations of a random sample against the oracle
-    idx = np.sort(np.random.default_rng(5).choice(n, 256, replace=False))
+    idx = np.sort(np.random.default_rng(5).choice(n, 1024, replace=False))
     t = time.time()
     rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, 0.0, 0.0, targets=idx)
     num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
@@ -249,59 +248,58 @@ def test_4m_workloads_full_size_sampled_against_oracle(name, nc, sigma, mass, se
     off = eng.offsets().astype(np.int64)
     ref = orc.summarize(off, h["px"], h["py"], h["mass"])
     np.testing.assert_allclose(eng.com(), ref, rtol=1e-12, atol=1e-15)
-    idx = np.sort(np.random.default_rng(6).choice(n, 512, replace=False))
+    idx = np.sort(np.random.default_rng(6).choice(n, 4096, replace=False))
     rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, 0.0, 0.0, targets=idx)
     num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
+    per = num / np.hypot(rx, ry)
     norm = float(np.sqrt((num ** 2).sum() / (np.hypot(rx, ry) ** 2).sum()))
-    print(f"[{name}] sampled normwise {norm:.3e}, p99 "
-          f"{np.quantile(num / np.hypot(rx, ry), 0.99):.3e}", flush=True)
+    print(f"[{name}] 4096 sampled particles: normwise {norm:.3e}, p99 "
+          f"{np.quantile(per, 0.99):.3e}, max {per.max():.3e}", flush=True)
     assert norm <= 1e-5
+    assert per.max() <= 1e-5
     eng.close()
 
 
-def test_bench_workload_fp32_trajectory_diverges_like_a_1e7_law_perturbation():
+def test_bench_workload_fp32_trajectory_matches_fp64_over_10_steps():
     """The bench workload at full size (4,194,304 particles, 64 clusters,
-    1024 x 1024 grid, G = 1e-6, dt = 1e-3, no softening), 10 full steps on the
-    FP32 path (with the FP64 recompute), on the FP64 path, and on the FP64 path
-    with G multiplied by 1 + 1e-7 (an exact, uniform 1e-7 force perturbation).
-    After one step the FP32 velocities match FP64 to the force error (~1e-7).
-    Later, unresolved close pairs (dt far above their orbital time, no
-    softening) amplify any difference and move particles across cell
-    boundaries, which changes COMs for everyone: the FP64 path perturbed by
-    1e-7 diverges from itself by ~1e-2 normwise after 10 steps (measured on
-    MI355X: 9.5e-3, median 1.6e-5; a 1e-10 perturbation gives 5.7e-4).  The
-    FP32 path may not diverge more than that perturbed FP64 path does."""
-    import torch
+    1024 x 1024 grid, G = 1e-6) under the bench's resolved law and step
+    (softening eps = 1e-5, dt = 1e-5): 10 full steps on the FP32 path (with
+    the FP64 recompute) and on the FP64 path.  Particles move ~1e-8 of the
+    box, so both paths place the same cells every step and the trajectories
+    differ only by the force error: after 10 steps every particle's velocity
+    and displacement agree to the per-particle target (1e-5), normwise to
+    ~1e-7.  (Without softening a few close pairs are never resolved: at
+    dt = 1e-5 some are still flung across cells within 10 steps.)"""
     n, G = 4194304, 1024
     d = inputs.gaussian_mixture(n, 64, (0.005, 0.05), seed=2024)
-    runs = {"fp64": ("fp64", GC), "fp64_law": ("fp64", GC * (1 + 1e-7)), "fp32": ("fp32", GC)}
-    vel = {}
-    for name, (prec, gc) in runs.items():
+    out = {}
+    for prec in ("fp64", "fp32"):
         eng = _engine(n, G)
         eng.load(d)
-        vel[name] = []
         for s in range(10):
-            eng.step(gc, 0.0, 0.0, prec, DT, graph=True)
-            if s in (0, 9):
-                h = eng.fetch()
-                o = np.argsort(h["pid"])
-                vel[name].append((h["vx"][o], h["vy"][o]))
+            eng.step(GC, BENCH_EPS2, 0.0, prec, BENCH_DT, graph=True)
         eng.check()
+        h = eng.fetch()
+        o = np.argsort(h["pid"])
+        out[prec] = {k: h[k][o] for k in ("px", "py", "vx", "vy")}
+        print(f"[trajectory] {prec} done", flush=True)
         eng.close()
-
-    def norm(a, b):
-        return float(np.sqrt(((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2).sum()
-                             / (b[0] ** 2 + b[1] ** 2).sum()))
-
-    def med(a, b):
-        return float(np.median(np.hypot(a[0] - b[0], a[1] - b[1]) / np.hypot(b[0], b[1])))
-
-    one = norm(vel["fp32"][0], vel["fp64"][0])
-    d32, dlaw = norm(vel["fp32"][1], vel["fp64"][1]), norm(vel["fp64_law"][1], vel["fp64"][1])
-    m32, mlaw = med(vel["fp32"][1], vel["fp64"][1]), med(vel["fp64_law"][1], vel["fp64"][1])
-    print(f"[trajectory] step 1: fp32 vs fp64 {one:.2e}; step 10: fp32 vs fp64 {d32:.2e} "
-          f"(median {m32:.1e}), fp64 with G(1+1e-7) vs fp64 {dlaw:.2e} (median {mlaw:.1e})",
-          flush=True)
-    assert one <= 1e-6
-    assert d32 <= 2.0 * dlaw
-    assert m32 <= 2.0 * mlaw
+    a, b = out["fp32"], out["fp64"]
+    dv = np.hypot(a["vx"] - b["vx"], a["vy"] - b["vy"])
+    vv = np.hypot(b["vx"], b["vy"])
+    # displacements from the initial positions (the positions themselves
+    # agree to ~1e-16 relative, which says nothing)
+    ddx = np.hypot((a["px"] - d["px"]) - (b["px"] - d["px"]), (a["py"] - d["py"]) - (b["py"] - d["py"]))
+    dd = np.hypot(b["px"] - d["px"], b["py"] - d["py"])
+    ok = vv > 0
+    rel_v = dv[ok] / vv[ok]
+    rel_x = ddx[ok] / dd[ok]
+    norm_v = float(np.sqrt((dv ** 2).sum() / (vv ** 2).sum()))
+    print(f"[trajectory] 10 steps fp32 vs fp64: velocity normwise {norm_v:.2e}, p99 "
+          f"{np.quantile(rel_v, 0.99):.2e}, max {rel_v.max():.2e}; displacement max "
+          f"{rel_x.max():.2e}; largest displacement {dd.max():.2e}", flush=True)
+    assert ok.all()
+    assert dd.max() < 1e-5   # resolved: nothing moves a sizeable part of a cell (1e-3)
+    assert norm_v <= 1e-6
+    assert rel_v.max() <= 1e-5
+    assert rel_x.max() <= 1e-5

@@ -144,21 +144,57 @@ LAWS = [("gpu_ref", 1.0, 0.0, 0.0), ("cpu_ref_cut", 1.0, 0.0, 1e-5),
         ("softened_eps1e-3", 1.0, 1e-6, 0.0), ("G2_soft_cut", 2.5, 1e-6, 1e-5)]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp64_ieee"])
 @pytest.mark.parametrize("law", LAWS, ids=[l[0] for l in LAWS])
-def test_forces_fp64(law):
+def test_forces_fp64(law, prec):
+    """Both FP64 kernels (refined hardware reciprocal, IEEE division) against
+    the oracle under the four law settings."""
     _, Gc, eps2, r2 = law
     d = inputs.uniform(16384, seed=21)
     G = 32
     eng = _forces_setup(d, G)
-    eng.evaluate(Gc, eps2, r2, "fp64")
+    eng.evaluate(Gc, eps2, r2, prec)
     eng.check()
     h, com, rx, ry = _oracle_forces(eng, G, Gc, eps2, r2)
     g = eng.fetch()
     norm, per = _rel(g["ax"], g["ay"], rx, ry)
-    print(f"[fp64 {law[0]}] normwise={norm:.3e} max={per.max():.3e}", flush=True)
+    print(f"[{prec} {law[0]}] normwise={norm:.3e} max={per.max():.3e}", flush=True)
     assert norm <= 1e-12
     assert per.max() <= 1e-9
     eng.close()
+
+
+@pytest.mark.parametrize("law", LAWS, ids=[l[0] for l in LAWS])
+def test_fp64_refined_reciprocal_matches_ieee_division(law):
+    """The default FP64 kernel forms each weight m / q from v_rcp_f64 and one
+    third-order refinement (within about one ulp of the quotient); the
+    fp64_ieee kernel divides.  On a clustered set with masses over six decades
+    the two agree to a few ulp per term: normwise <= 1e-14 and every particle
+    within 1e-14 of the sum of its term magnitudes (checked as <= 1e-10
+    relative to |a| where the terms nearly cancel)."""
+    _, Gc, eps2, r2 = law
+    rng = np.random.default_rng(23)
+    if eps2 == 0.0 and r2 == 0.0:   # clipping can stack particles on a corner: 0/0 there
+        d = inputs.uniform(20000, seed=23)
+    else:
+        d = inputs.gaussian_mixture(20000, 6, (0.01, 0.1), seed=23)
+    d["mass"] = np.exp(rng.uniform(np.log(1e-3), np.log(1e3), len(d["px"])))
+    G = 40
+    out = {}
+    for prec in ("fp64", "fp64_ieee"):
+        eng = _forces_setup(d, G)
+        eng.evaluate(Gc, eps2, r2, prec)
+        eng.check()
+        out[prec] = eng.fetch()
+        eng.close()
+    f, i = out["fp64"], out["fp64_ieee"]
+    np.testing.assert_array_equal(f["pid"], i["pid"])
+    norm, per = _rel(f["ax"], f["ay"], i["ax"], i["ay"])
+    print(f"[fp64 vs ieee {law[0]}] normwise={norm:.3e} p99={np.quantile(per, 0.99):.3e} "
+          f"max={per.max():.3e}", flush=True)
+    assert norm <= 1e-14
+    assert np.quantile(per, 0.99) <= 1e-13
+    assert per.max() <= 1e-10
 
 
 @pytest.mark.parametrize("law", LAWS, ids=[l[0] for l in LAWS])
@@ -330,7 +366,8 @@ def test_tick_bit_exact():
     eng.close()
 
 
-def test_fp64_forces_independent_of_tuning():
+@pytest.mark.parametrize("prec", ["fp64", "fp64_ieee"])
+def test_fp64_forces_independent_of_tuning(prec):
     """FP64 sums run in a fixed order per particle: bit-identical across
     launch shapes and repeated runs."""
     d = inputs.gaussian_mixture(30000, 8, (0.01, 0.05), seed=9)
@@ -339,7 +376,7 @@ def test_fp64_forces_independent_of_tuning():
     for tun in ({"p": 1, "yspan": 1, "waves_per_cu": 4}, {"p": 2, "yspan": 16, "waves_per_cu": 16},
                 {"p": 4, "yspan": 64, "waves_per_cu": 8}, {"p": 2, "yspan": 16, "waves_per_cu": 16}):
         eng = _forces_setup(d, G, tun)
-        eng.evaluate(1.0, 1e-6, 0.0, "fp64")
+        eng.evaluate(1.0, 1e-6, 0.0, prec)
         g = eng.fetch()
         if ref is None:
             ref = g

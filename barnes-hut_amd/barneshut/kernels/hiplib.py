@@ -37,8 +37,11 @@ EXPORTED = (
     "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy", "bh_set_recompute",
-    "bh_recompute",
+    "bh_recompute", "bh_step_noalloc",
 )
+# limits of the allocation-free step (bh_step_noalloc)
+NOALLOC_MAX_N = 262144
+NOALLOC_MAX_GRID = 256
 
 
 class BhError(RuntimeError):
@@ -119,6 +122,8 @@ def load(path: str = LIB_PATH):
         "bh_energy": (ctypes.c_int, [vp, P(BhParticles), dbl, dbl, P(dbl), vp]),
         "bh_set_recompute": (ctypes.c_int, [vp, dbl, i32]),
         "bh_recompute": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), vp]),
+        "bh_step_noalloc": (ctypes.c_int, [P(BhParticles), P(BhParticles), i64, i32, P(BhLaw),
+                                           i32, dbl, i32, P(BhTuning), dbl, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -384,6 +389,29 @@ class Context:
         self._params(params)
         self._order(order, optional=False)
         check(self.L.bh_learned_order(self.h, ptr(params), ptr(order), stream_handle(stream)))
+
+
+def step_noalloc(a, b, grid_dim, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", dt=0.0,
+                 do_tick=True, tuning=None, kappa=2.0, stats=None, stream=None):
+    """bh_step_noalloc: one full round of one system (a placed and sorted
+    into b, then summarize, forces, tick on b) with no device allocation; all
+    scratch comes from the library's static per-GPU array.  tuning: dict of
+    bh_tuning fields (tail_split must be 1) or None; stats: device int32[16]
+    or None."""
+    n = a["px"].numel()
+    pa, pb = particles_struct(a, n), particles_struct(b, n)
+    device_buffer(stats, "stats", "int32", 16, optional=True)
+    law = BhLaw(float(G), float(eps2), float(r2_cut))
+    prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    tun = None
+    if tuning is not None:
+        tun = ctypes.byref(BhTuning(int(tuning["p"]), int(tuning["yspan"]),
+                                    int(tuning["waves_per_cu"]), int(tuning["tail_split"]),
+                                    int(tuning["tail_items"])))
+    L = load()
+    check(L.bh_step_noalloc(ctypes.byref(pa), ctypes.byref(pb), int(n), int(grid_dim),
+                            ctypes.byref(law), prec, float(dt), 1 if do_tick else 0, tun,
+                            float(kappa), ptr(stats), stream_handle(stream)))
 
 
 def _mlp_rows(x, name="x"):

@@ -5,7 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bh_hip.h"
@@ -133,6 +135,7 @@ static int dalloc(Ctx* c, T** p, int64_t count)
 
 static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 static int ensure_part(Ctx* c);
+static int scratch_address(int dev, unsigned char** out, int* num_cu);
 static inline unsigned grid_for(int64_t n, int bs, int64_t cap = 1 << 20)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, bs), cap));
@@ -294,6 +297,14 @@ int bh_create(int device, void** ctx_out)
         hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming) != hipSuccess) {
         delete c;
         return fail(BH_EHIP, "stream/event creation failed");
+    }
+    // the allocation-free step's static scratch: its device address on this GPU
+    unsigned char* sp = nullptr;
+    int r = scratch_address(device, &sp, nullptr);
+    if (r) {
+        const std::string msg = g_err;
+        bh_destroy(c);
+        return fail(r, msg);
     }
     *ctx_out = c;
     return BH_OK;
@@ -1021,6 +1032,139 @@ int bh_interactions(void* ctx, int64_t* out, void* stream)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free step
+// bh_step_noalloc: bh_step for one system with every per-step buffer carved
+// out of one static device array per GPU, `scratch`, so the call makes no
+// hipMalloc / hipFree (nor any other allocating call).  The array keeps the
+// last call's contents; the setup kernel clears what must start at zero.
+// Calls that overlap on one GPU share the array and race.
+static __device__ __align__(256) unsigned char scratch[BH_NOALLOC_SCRATCH_BYTES];
+
+#define NOALLOC_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_addr[NOALLOC_MAX_DEVICES];   // device address of scratch
+static int g_scratch_cu[NOALLOC_MAX_DEVICES];
+
+// Device address of `scratch` on device dev (which must be current): looked up
+// once per device with hipGetSymbolAddress (each GPU holds its own copy of the
+// array at its own address; the array's name is only a host-side stand-in).
+static int scratch_address(int dev, unsigned char** out, int* num_cu)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(BH_EINVAL, "device " + std::to_string(dev) + " out of range for the static scratch");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(BH_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        int cu = 0;
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cu <= 0)
+            cu = 256;
+        g_scratch_cu[dev] = cu;
+        g_scratch_addr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_addr[dev];
+    if (num_cu) *num_cu = g_scratch_cu[dev];
+    return BH_OK;
+}
+
+// The one system's descriptor, every row in system 0, and the state a step
+// reads before writing (stats, offsets, non-empty prefix) cleared.
+__global__ void k_noalloc_setup(SysDev* sys, int32_t* sys_of, int64_t n, int32_t G, int32_t Gb,
+                                int32_t* stats, int32_t* offsets, int32_t* neprefix, int64_t C)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    if (t == 0) {
+        SysDev d{};
+        d.n = n;
+        d.G = G;
+        d.Gb = Gb;
+        sys[0] = d;
+    }
+    for (int64_t i = t; i < n; i += stride) sys_of[i] = 0;
+    for (int64_t i = t; i <= C; i += stride) {
+        offsets[i] = 0;
+        neprefix[i] = 0;
+    }
+    for (int64_t i = t; i < ST_COUNT; i += stride) stats[i] = 0;
+}
+
+__global__ void k_copy_i32(const int32_t* src, int32_t* dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+extern "C" int bh_step_noalloc(const bh_particles* a, const bh_particles* b, int64_t n,
+                               int32_t grid_dim, const bh_law* law, int32_t precision, double dt,
+                               int32_t do_tick, const bh_tuning* tuning, double kappa,
+                               int32_t* stats, void* stream)
+{
+    if (n < 1 || n > BH_NOALLOC_MAX_N)
+        return fail(BH_EINVAL, "bh_step_noalloc: n = " + std::to_string(n) +
+                                   " outside [1, " + std::to_string(BH_NOALLOC_MAX_N) +
+                                   "], the static scratch's limit");
+    if (grid_dim < 1 || grid_dim > BH_NOALLOC_MAX_GRID)
+        return fail(BH_EINVAL, "bh_step_noalloc: grid_dim = " + std::to_string(grid_dim) +
+                                   " outside [1, " + std::to_string(BH_NOALLOC_MAX_GRID) +
+                                   "], the static scratch's limit");
+    bh_tuning t = {2, 512, 32, 1, 0};
+    if (tuning) t = *tuning;
+    if (!(t.p == 1 || t.p == 2 || t.p == 4) || t.yspan < 1 || t.yspan > (1 << 20) ||
+        t.waves_per_cu < 1 || t.waves_per_cu > 64 || t.tail_items < 0)
+        return fail(BH_EINVAL, "bh_step_noalloc: invalid tuning");
+    if (t.tail_split != 1)
+        return fail(BH_EINVAL, "bh_step_noalloc: tail_split must be 1 (the slice planes do not fit "
+                               "the static scratch)");
+    if (std::isnan(kappa)) return fail(BH_EINVAL, "bh_step_noalloc: kappa is NaN");
+    int r = check_parts(a, "a");
+    if (r) return r;
+    if ((r = check_parts(b, "b"))) return r;
+    if (a->px == b->px) return fail(BH_EINVAL, "a and b must be different buffers");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    unsigned char* base = nullptr;
+    int num_cu = 256;
+    if ((r = scratch_address(dev, &base, &num_cu))) return r;
+
+    Ctx c;   // host-side view: every device pointer points into scratch
+    c.device = dev;
+    c.num_cu = num_cu;
+    const int64_t G = grid_dim, Gb = cdiv(G, RC_BLOCK);
+    set_sizes(&c, 1, n, G * G, G, n, Gb * Gb);
+    size_t off = 0;
+    r = for_each_buffer(&c, [&](auto*& p, int64_t cnt) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        off = (off + 255) & ~(size_t)255;
+        const size_t bytes = (size_t)std::max<int64_t>(cnt, 1) * sizeof(T);
+        if (off + bytes > (size_t)BH_NOALLOC_SCRATCH_BYTES)
+            return fail(BH_EINVAL, "bh_step_noalloc: static scratch too small for this size");
+        p = (T*)(base + off);
+        off += bytes;
+        return (int)BH_OK;
+    });
+    if (r) return r;
+    c.tune = t;
+    c.rc_kappa = kappa > 0.0 ? kappa : 0.0;
+    c.rc_deferred = 0;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_noalloc_setup, dim3(grid_for(std::max<int64_t>(n, c.C + 1), 256, 1024)),
+                       dim3(256), 0, st, c.sys, c.sys_of, n, (int32_t)G, (int32_t)Gb, c.stats,
+                       c.offsets, c.neprefix, c.C);
+    LAUNCH_CHECK("noalloc setup");
+    r = step_impl(&c, a, b, law, precision, dt, do_tick, nullptr, nullptr, stream);
+    if (r) return r;
+    if (stats) {
+        hipLaunchKernelGGL(k_copy_i32, dim3(1), dim3(64), 0, st, (const int32_t*)c.stats, stats,
+                           (int)ST_COUNT);
+        LAUNCH_CHECK("noalloc stats");
+    }
+    return BH_OK;
+}
 
 // Accessors for the cost-model translation unit.
 Item* bh_ctx_items(void* ctx) { return ((Ctx*)ctx)->items; }

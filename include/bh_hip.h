@@ -219,6 +219,32 @@ int bh_learned_order(void* ctx, const float* params, int32_t* order, void* strea
  * 4 work-item capacity, 5 compute units. */
 int bh_query(void* ctx, int32_t what, int64_t* out);
 
+/* Allocation-free step.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  bh_step for ONE system of
+ * n particles on a grid_dim^2 grid, without a context: every per-step buffer
+ * is carved out of one static device array per GPU (BH_NOALLOC_SCRATCH_BYTES,
+ * held by each GPU the library runs on until the process ends), so the call
+ * makes no hipMalloc / hipFree.  Limits: 1 <= n <= BH_NOALLOC_MAX_N,
+ * 1 <= grid_dim <= BH_NOALLOC_MAX_GRID; beyond them BH_EINVAL.  a is placed
+ * and sorted into b, then summarize, evaluate (static order) and, if do_tick,
+ * tick, all on b -- the same kernels as bh_step on a context configured with
+ * bh_set_systems(1, &n, &grid_dim), bh_set_tuning(tuning) and
+ * bh_set_recompute(kappa, 0), and the same results bit for bit.
+ * tuning: NULL = {p 2, yspan 512, waves_per_cu 32, tail_split 1,
+ * tail_items 0}; tail_split must be 1.  kappa: FP64 recompute threshold of
+ * the FP32 precisions (<= 0: off).  stats: device int32[16] that receives
+ * the step's stats block (layout of bh_read_stats; [0] error bits), or NULL.
+ * Runs on the calling thread's current device, asynchronously on stream.
+ * Every call on one GPU shares that GPU's array (the last call's contents
+ * stay in it): calls that overlap on one GPU race. */
+#define BH_NOALLOC_MAX_N 262144
+#define BH_NOALLOC_MAX_GRID 256
+#define BH_NOALLOC_SCRATCH_BYTES (32 << 20)
+int bh_step_noalloc(const bh_particles* a, const bh_particles* b, int64_t n, int32_t grid_dim,
+                    const bh_law* law, int32_t precision, double dt, int32_t do_tick,
+                    const bh_tuning* tuning, double kappa, int32_t* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,11 +12,17 @@ _GPU_NAMES = ("singlegpu", "hip", "mi355x")
 
 class BarnesHut:
 
-    def __init__(self, ini_file):
+    def __init__(self, ini_file, concurrent=1):
+        """concurrent > 1: that many copies of the input, stepped together as
+        independent systems of one context (the form bin/run_2d_concurrent.py
+        uses for the reference's K concurrent BarnesHut objects)."""
         Config.read_file(ini_file)
         impl = Config.get("general", "implementation", fallback="sequential")
         print(f"Using {impl} implementation")
-        if impl in _GPU_NAMES:
+        if impl in _GPU_NAMES and concurrent > 1:
+            from .implementations.concurrent import ConcurrentHipBarnesHut
+            self.bh = ConcurrentHipBarnesHut(concurrent)
+        elif impl in _GPU_NAMES:
             from .implementations.hip import HipBarnesHut
             self.bh = HipBarnesHut()
         else:

@@ -17,14 +17,17 @@ Differences from the reference path, each deliberate:
     also avoids the reference kernel's 0/0 for a particle at (0, 0).
 
 Extra keys, section [hip]:
-  precision   fp64 (default) | fp32 | fp32_kahan_term | fp32_plain
+  precision   fp64 (default) | fp64_ieee | fp32 | fp32_kahan_term | fp32_plain
   eps2        softening added to r^2 (default 0 = reference GPU law)
   r2_cut      drop pairs with r^2 <= r2_cut (reference CPU law: 1e-5)
   grid_dim    explicit grid side (0: from max_particles_per_box)
   schedule    static (default) | learned
   cost_model  .npz file of a trained cost model (schedule = learned)
-  p, yspan, waves_per_cu, tail_split, tail_items   force-kernel launch shape
-              (autotune = True to tune)
+  p, yspan, waves_per_cu, tail_split, tail_items   force-kernel launch shape of
+              the FP32 precisions (autotune = True to tune)
+  p64, yspan64, waves_per_cu64   launch shape of the FP64 precisions (default
+              2, 512, 32: the fastest on the 4M-particle workloads; FP64 items
+              are never split, so no slice planes are allocated)
   recompute_kappa  FP32 precisions: FP64 recompute of the particles whose
               rounding-error bound exceeds kappa * |a| (default 2; 0 = off)
 """
@@ -51,6 +54,12 @@ class HipBarnesHut(BaseBarnesHut):
                            waves_per_cu=int(Config.get("hip", "waves_per_cu", fallback="32")),
                            tail_split=int(Config.get("hip", "tail_split", fallback="16")),
                            tail_items=int(Config.get("hip", "tail_items", fallback="1048576")))
+        if self.precision in ("fp64", "fp64_ieee"):
+            self.tuning = dict(p=int(Config.get("hip", "p64", fallback="2")),
+                               yspan=int(Config.get("hip", "yspan64", fallback="512")),
+                               waves_per_cu=int(Config.get("hip", "waves_per_cu64",
+                                                           fallback="32")),
+                               tail_split=1, tail_items=0)
         self.device = int(Config.get("hip", "device", fallback="0"))
         self.recompute_kappa = float(Config.get("hip", "recompute_kappa", fallback="2.0"))
         self.engine = None

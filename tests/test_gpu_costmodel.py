@@ -161,44 +161,39 @@ def test_collect_train_rank_on_logged_costs():
     e.close()
 
 
-def _force_ms(e, order, reps=3):
-    import torch
-    best = float("inf")
-    for _ in range(reps):
-        e.parts["ax"].zero_()
-        e.parts["ay"].zero_()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        e.ctx.force_pass(e.parts, 1e-6, 0.0, 0.0, "fp32", order)
-        b.record()
-        b.synchronize()
-        best = min(best, a.elapsed_time(b))
-    return best
+def _batched_engine(sizes, seed):
+    from barneshut.batched import BATCHED_TUNING
+    from barneshut.engine import Engine
+    d = inputs.batched_uniform(sizes, seed=seed)
+    grids = np.ceil(np.sqrt(np.asarray(sizes) / 16.0)).astype(np.int32)
+    e = Engine(sizes, grids, tuning=BATCHED_TUNING)
+    e.load(d)
+    return e
 
 
-@pytest.mark.parametrize("split", [8, 1], ids=["sliced", "whole_items"])
-def test_trained_model_gain_over_static(split):
-    """What the shipped model (trained on 256 clusters, lognormal masses) gains
-    over the static queue order on that kind of input, with the default
-    far-field slicing and with whole items (where the order alone decides the
-    tail).  Reported; asserted only not to lose more than timing noise."""
+def _batched_model():
     import os
     import torch
     from conftest import PKG
-    d = inputs.gaussian_mixture(1 << 20, 256, (0.001, 0.03), seed=12, mass="lognormal")
-    tun = dict(p=2, yspan=512, waves_per_cu=32, tail_split=split,
-               tail_items=(1 << 20) if split > 1 else 0)
-    e = _engine(d, 512, tun)
-    model = costmodel.CostModel.load(os.path.join(PKG, "costmodels", "clusters256.npz"),
-                                     device=torch.device("cuda"))
-    order = torch.empty(e.ctx.query(4), dtype=torch.int32, device="cuda")
-    e.ctx.learned_order(model.params, order)
-    static = _force_ms(e, None)
-    learned = _force_ms(e, order)
-    print(f"[gain split={split}] static {static:.3f} ms, learned {learned:.3f} ms, "
-          f"gain {static / learned:.4f}", flush=True)
-    assert learned <= 1.03 * static
+    return costmodel.CostModel.load(os.path.join(PKG, "costmodels", "batched512.npz"),
+                                    device=torch.device("cuda"))
+
+
+def test_batched_model_ranks_logged_item_costs():
+    """The shipped batched model against item durations the force kernel logs
+    on a batch it was not trained on (64 uneven systems, another seed): the
+    rank correlation of predicted and measured costs must clear 0.4 (a model
+    that predicts noise scores ~0; measured 0.59 on MI355X).  Items of one system share their
+    far-field length K, so the signal is K across systems plus the near field
+    within them."""
+    sizes, _ = inputs.batched_sizes(64, 1024, 262144, seed=77)
+    e = _batched_engine(sizes, 77)
+    X, c = costmodel.collect(e, 2, (1e-6, 1e-12, 0.0), "fp32", dt=1e-5)
+    model = _batched_model()
+    pred = model.predict(_t(X)).cpu().numpy()
+    rq = costmodel.rank_quality(pred, c)
+    print(f"[rank] {len(c)} logged items, rank correlation {rq:.3f}", flush=True)
+    assert rq >= 0.4
     e.close()
 
 
@@ -206,8 +201,12 @@ def test_trained_model_gain_on_small_batches():
     """Where the queue is short against the resident waves -- batches of 32
     systems of uneven size, a few milliseconds of force work -- the order of
     the items decides the tail, and the shipped batched model's
-    longest-first order pays off (measured 1.00-1.13x per batch on MI355X).
-    Reported per batch; asserted only not to lose on average."""
+    longest-first order pays off: measured 1.003, 1.047 and 1.072 (geometric
+    mean 1.041) on MI355X, 1.012, 1.031 and 1.108 on another box.  The
+    geometric mean must reach 1.01.  (Elsewhere the order gains nothing
+    measurable: with 8 waves per SIMD the last long items run faster as the
+    other waves drain, so even a batch sorted smallest-first loses < 1 % in
+    the static order.)"""
     import os
     import torch
     from conftest import PKG
@@ -243,4 +242,4 @@ def test_trained_model_gain_on_small_batches():
         e.close()
     gm = float(np.exp(np.mean(np.log(gains))))
     print(f"[small batches] geometric-mean gain {gm:.3f}", flush=True)
-    assert gm >= 0.98
+    assert gm >= 1.01

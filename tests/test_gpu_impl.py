@@ -64,6 +64,61 @@ def test_run_2d_cli_with_check_equals_direct_sum_on_2x2_grid(tmp_path):
     bh.bh.close()
 
 
+def _load_bin(name):
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        name, os.path.join(ROOT, "barnes-hut_amd", "bin", f"{name}.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _oracle_steps(d, G, steps, Gc=1.0, dt=0.005, eps2=1e-6):
+    n = len(d["px"])
+    st = dict(d, ax=np.zeros(n), ay=np.zeros(n))
+    for _ in range(steps):
+        o = orc.step(st["px"], st["py"], st["mass"], st["vx"], st["vy"], st["pid"], G, Gc, dt,
+                     eps2, 0.0, st["ax"], st["ay"])
+        inv = np.argsort(o["pid"])
+        st = dict(px=o["px_next"][inv], py=o["py_next"][inv], mass=o["m"][inv],
+                  vx=o["vx_next"][inv], vy=o["vy_next"][inv], pid=o["pid"][inv],
+                  ax=np.zeros(n), ay=np.zeros(n))
+    return st
+
+
+def test_run_2d_concurrent_cli_copies_equal_single_system_and_oracle(tmp_path, capsys):
+    """run_2d_concurrent.py input_file nwarmups nruns concurrent configfile
+    --check (the reference's CLI, bin/run_2d_concurrent.py:29-90): four copies
+    of one input stepped as four systems of one context.  Every copy equals
+    system 0 bit for bit after every run; system 0 equals the single-system
+    drop-in (run_2d.py with the same INI and runs) bit for bit and tracks the
+    oracle over the 6 steps taken (1 warm-up + 2 runs of 2 rounds); on a 2x2
+    grid the check against the direct sum is exact."""
+    run_c = _load_bin("run_2d_concurrent")
+    run_1 = _load_bin("run_2d")
+    fin, d = _input(tmp_path, 3000, seed=7)
+    ini = _ini(tmp_path, rounds=2, eps2=1e-6, grid_dim=12)
+    bhc = run_c.main([fin, "1", "2", "4", ini, "--check"])
+    out = capsys.readouterr().out
+    assert out.count("4 concurrent systems identical: True") == 3
+    assert "warmup done" in out and "end-to-end" in out and "evaluation" in out
+    assert bhc.bh.systems_identical()
+    got = bhc.bh.get_particles()
+    bhc.bh.close()
+    bh1 = run_1.main([fin, "1", "2", ini, "--check"])
+    one = bh1.bh.get_particles()
+    bh1.bh.close()
+    np.testing.assert_array_equal(got, one)
+    st = _oracle_steps(d, 12, 6)
+    np.testing.assert_array_equal(got[:, 9], np.arange(3000))
+    np.testing.assert_allclose(got[:, 0], st["px"], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(got[:, 3], st["vx"], rtol=1e-7, atol=1e-9)
+    ini2 = _ini(tmp_path, rounds=1, eps2=1e-6, grid_dim=2)
+    bh2 = run_c.main([fin, "0", "1", "3", ini2, "--check"])
+    assert max(bh2.bh.last_check["vector_rel"]) < 1e-10
+    bh2.bh.close()
+
+
 def test_rounds_track_oracle_steps(tmp_path):
     fin, d = _input(tmp_path, 4000, seed=2)
     G = orc.grid_dim(4000, 1000)
@@ -111,14 +166,22 @@ def test_fp32_precision_through_ini(tmp_path):
     fin, d = _input(tmp_path, 20000, seed=4)
     from barneshut import BarnesHut
     r = {}
-    for prec in ("fp64", "fp32"):
+    shapes = {}
+    for prec in ("fp64", "fp32", "fp64_ieee"):
         bh = BarnesHut(_ini(tmp_path, precision=prec, skip=True, grid_dim=64))
         bh.read_input_file(fin)
         bh.run()
         r[prec] = bh.bh.get_particles()[:, 5:7].copy()
+        shapes[prec] = bh.bh.engine.ctx.get_tuning()
         bh.bh.close()
     num = np.linalg.norm(r["fp32"] - r["fp64"])
     assert num / np.linalg.norm(r["fp64"]) < 1e-5
+    assert np.linalg.norm(r["fp64_ieee"] - r["fp64"]) <= 1e-14 * np.linalg.norm(r["fp64"])
+    # the FP64 precisions run their own launch shape ([hip] p64, yspan64,
+    # waves_per_cu64), never split into slices; FP32 the [hip] p ... shape
+    fp64_shape = dict(p=2, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0)
+    assert shapes["fp64"] == fp64_shape and shapes["fp64_ieee"] == fp64_shape
+    assert shapes["fp32"]["tail_split"] == 16
 
 
 def test_nonfinite_coordinate_raises(tmp_path):

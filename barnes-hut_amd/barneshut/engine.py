@@ -137,6 +137,14 @@ class Engine:
         and kinetic energy."""
         return self.ctx.energy(self.parts, G, eps2)
 
+    def energy_grid(self, G=1.0, eps2=0.0):
+        """[(U_s, K_s)] per system with the potential evaluated by the grid
+        method (about one FP32 force pass of work): placement and summary of
+        the current state first, then the potential kernel."""
+        self.create_tree()
+        self.summarize()
+        return self.ctx.energy_grid(self.parts, G, eps2)
+
     def interactions(self):
         """(near, far) per system and total, exact, from the current counts."""
         out = self.ctx.interactions()

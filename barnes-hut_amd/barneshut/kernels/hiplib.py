@@ -37,7 +37,7 @@ EXPORTED = (
     "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy", "bh_set_recompute",
-    "bh_recompute", "bh_step_noalloc",
+    "bh_recompute", "bh_step_noalloc", "bh_energy_grid",
 )
 # limits of the allocation-free step (bh_step_noalloc)
 NOALLOC_MAX_N = 262144
@@ -120,6 +120,7 @@ def load(path: str = LIB_PATH):
         "bh_learned_order": (ctypes.c_int, [vp, vp, vp, vp]),
         "bh_query": (ctypes.c_int, [vp, i32, P(i64)]),
         "bh_energy": (ctypes.c_int, [vp, P(BhParticles), dbl, dbl, P(dbl), vp]),
+        "bh_energy_grid": (ctypes.c_int, [vp, P(BhParticles), dbl, dbl, P(dbl), vp]),
         "bh_set_recompute": (ctypes.c_int, [vp, dbl, i32]),
         "bh_recompute": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), vp]),
         "bh_step_noalloc": (ctypes.c_int, [P(BhParticles), P(BhParticles), i64, i32, P(BhLaw),
@@ -367,6 +368,17 @@ class Context:
         check(self.L.bh_energy(self.h, ctypes.byref(p), float(G), float(eps2),
                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                stream_handle(stream)))
+        return out.reshape(-1, 2)
+
+    def energy_grid(self, parts, G=1.0, eps2=0.0, stream=None):
+        """[(U_s, K_s)] per system with the potential by the grid method (near
+        cells direct, far cells from their COMs; needs the current placement
+        and summary of parts)."""
+        p = self._parts(parts)
+        out = np.zeros(2 * self.n_sys, np.float64)
+        check(self.L.bh_energy_grid(self.h, ctypes.byref(p), float(G), float(eps2),
+                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    stream_handle(stream)))
         return out.reshape(-1, 2)
 
     def interactions_async(self, dev_out, stream=None):

@@ -85,7 +85,7 @@ struct bh_ctx_s {
     bh_tuning tune = {2, 512, 32, 16, 1 << 20};
     double* part = nullptr;
     size_t part_cap = 0;
-    double* energy_buf = nullptr;   // 2 x N, allocated by the first bh_energy
+    double* energy_buf = nullptr;   // 2 x N + S, allocated by the first energy call
     // FP64 recompute: threshold, the FP32 pass's per-particle scratch, the
     // flagged list and groups, per-block sums of M^2
     double rc_kappa = 2.0;
@@ -965,30 +965,18 @@ int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream)
     return BH_OK;
 }
 
-int bh_energy(void* ctx, const bh_particles* p, double G, double eps2, double* out, void* stream)
+}  // extern "C"
+
+// Per system: out[2s] = G / 4 * sum u_i and out[2s+1] = sum k_i / 2 over the
+// system's rows of energy_buf (u: N values, then k: N values), Neumaier sums in
+// row order on the host.  Synchronises.
+static int energy_sums(Ctx* c, double G, double* out, hipStream_t st)
 {
-    Ctx* c = (Ctx*)ctx;
-    CTX_READY(c);
-    if (!out) return fail(BH_EINVAL, "out is NULL");
-    if (int e = check_parts(p, "energy")) return e;
-    if (!c->energy_buf)
-        if (int e = dalloc(c, &c->energy_buf, 2 * c->N)) return e;
-    hipStream_t st = (hipStream_t)stream;
-    double* u = c->energy_buf;
-    double* k = c->energy_buf + c->N;
-    for (int s = 0; s < c->S; ++s) {
-        const SysDev& d = c->sys_h[s];
-        if (d.n == 0) continue;
-        hipLaunchKernelGGL(k_energy, dim3((unsigned)cdiv(d.n, 256)), dim3(256), 0, st, p->px, p->py,
-                           p->mass, p->vx, p->vy, d.p_off, d.n, eps2, u, k);
-    }
-    LAUNCH_CHECK("energy");
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<double> h(2 * c->N);
     HIP_TRY(hipMemcpy(h.data(), c->energy_buf, sizeof(double) * 2 * c->N, hipMemcpyDeviceToHost));
     for (int s = 0; s < c->S; ++s) {
         const SysDev& d = c->sys_h[s];
-        // Neumaier sums in row order
         double su = 0.0, cu = 0.0, sk = 0.0, ck = 0.0;
         for (int64_t i = d.p_off; i < d.p_off + d.n; ++i) {
             double v = h[i], t = su + v;
@@ -1003,6 +991,72 @@ int bh_energy(void* ctx, const bh_particles* p, double G, double eps2, double* o
         out[2 * s + 1] = 0.5 * (sk + ck);
     }
     return BH_OK;
+}
+
+extern "C" {
+
+int bh_energy(void* ctx, const bh_particles* p, double G, double eps2, double* out, void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!out) return fail(BH_EINVAL, "out is NULL");
+    if (int e = check_parts(p, "energy")) return e;
+    if (!c->energy_buf)
+        if (int e = dalloc(c, &c->energy_buf, 2 * c->N + c->S)) return e;
+    hipStream_t st = (hipStream_t)stream;
+    double* u = c->energy_buf;
+    double* k = c->energy_buf + c->N;
+    for (int s = 0; s < c->S; ++s) {
+        const SysDev& d = c->sys_h[s];
+        if (d.n == 0) continue;
+        hipLaunchKernelGGL(k_energy, dim3((unsigned)cdiv(d.n, 256)), dim3(256), 0, st, p->px, p->py,
+                           p->mass, p->vx, p->vy, d.p_off, d.n, eps2, u, k);
+    }
+    LAUNCH_CHECK("energy");
+    return energy_sums(c, G, out, st);
+}
+
+int bh_energy_grid(void* ctx, const bh_particles* p, double G, double eps2, double* out,
+                   void* stream)
+{
+    Ctx* c = (Ctx*)ctx;
+    CTX_READY(c);
+    if (!out) return fail(BH_EINVAL, "out is NULL");
+    if (int e = check_parts(p, "energy")) return e;
+    if (!(eps2 >= 0.0) || !std::isfinite(G)) return fail(BH_EINVAL, "need finite G and eps2 >= 0");
+    if (!c->energy_buf)
+        if (int e = dalloc(c, &c->energy_buf, 2 * c->N + c->S)) return e;
+    hipStream_t st = (hipStream_t)stream;
+    if (int e = bh_build_items(ctx, stream)) return e;
+    double* u = c->energy_buf;
+    double* msys = c->energy_buf + 2 * c->N;
+    hipLaunchKernelGGL(k_sys_mass, dim3((unsigned)c->S), dim3(256), 0, st, (const SysDev*)c->sys,
+                       (const int32_t*)c->neprefix, (const Com64*)c->l64, msys);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
+    PotArgs pa;
+    pa.items = c->items;
+    pa.stats = c->stats;
+    pa.sys = c->sys;
+    pa.offsets = c->offsets;
+    pa.neprefix = c->neprefix;
+    pa.key_sorted = c->key_sorted;
+    pa.x = p->px;
+    pa.y = p->py;
+    pa.m = p->mass;
+    pa.vx = p->vx;
+    pa.vy = p->vy;
+    pa.l64 = c->l64;
+    pa.l32 = c->l32;
+    pa.msys = msys;
+    pa.eps2 = eps2;
+    pa.u = u;
+    pa.k = u + c->N;
+    const dim3 grid((unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu));
+    if (c->tune.p == 1) hipLaunchKernelGGL(k_potential<1>, grid, dim3(64), 0, st, pa);
+    else if (c->tune.p == 2) hipLaunchKernelGGL(k_potential<2>, grid, dim3(64), 0, st, pa);
+    else hipLaunchKernelGGL(k_potential<4>, grid, dim3(64), 0, st, pa);
+    LAUNCH_CHECK("energy_grid");
+    return energy_sums(c, G, out, st);
 }
 
 int bh_interactions(void* ctx, int64_t* out, void* stream)

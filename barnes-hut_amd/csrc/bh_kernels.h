@@ -44,6 +44,22 @@ struct ForceArgs {
     int rc;
 };
 
+// Arguments of the grid-method potential (bh_potential.hip).
+struct PotArgs {
+    const Item* items;
+    int32_t* stats;
+    const SysDev* sys;
+    const int32_t* offsets;
+    const int32_t* neprefix;
+    const int32_t* key_sorted;
+    const double *x, *y, *m, *vx, *vy;
+    const Com64* l64;
+    const Com32* l32;
+    const double* msys;   // total mass per system
+    double eps2;
+    double *u, *k;        // m_i phi_i and m_i |v_i|^2 per particle
+};
+
 // m / q to FP64 accuracy without the IEEE division sequence: v_rcp_f64 gives
 // r0 with e = 1 - q r0 small (|e| < 2^-22), and 1/q = r0 (1 + e + e^2 + e^3 + ...),
 // so m r0 (1 + e + e^2) differs from m / q by the relative e^3 plus the
@@ -83,6 +99,12 @@ __device__ __forceinline__ double near_w64(double wx, double wy, double m, doubl
     if (LAW == 0) return qdiv64<FAST>(m, s2 + eps2);
     return (s2 > r2cut) ? qdiv64<FAST>(m, s2 + eps2) : 0.0;
 }
+
+// grid-method potential
+__global__ void k_sys_mass(const SysDev* sys, const int32_t* neprefix, const Com64* l64,
+                           double* msys);
+template <int P>
+__global__ void k_potential(PotArgs A);
 
 // placement
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);

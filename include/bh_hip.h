@@ -190,6 +190,17 @@ int bh_interactions_async(void* ctx, uint64_t* dev_out, void* stream);
  * out[2s+1] = K_s = sum m v^2 / 2.  Rows of p in the context's row order.
  * Synchronises. */
 int bh_energy(void* ctx, const bh_particles* p, double G, double eps2, double* out, void* stream);
+/* The same energy with the potential evaluated by the grid method (no
+ * reference counterpart either): per system, out[2s] = G / 4 * sum_i m_i phi_i
+ * with phi_i = sum over the other particles of i's 3x3 cells of
+ * m_j log(r_ij^2 + eps2) plus sum over the other non-empty cells k of
+ * M_k log(|X_k - x_i|^2 + eps2) -- the force kernels' decomposition with the
+ * log in place of the reciprocal, evaluated in FP32 cell-relative terms
+ * (relative error ~1e-7), about one FP32 force pass of work -- and out[2s+1]
+ * = K_s.  p must be placed and summarized (bh_place_particles ...
+ * bh_summarize, or a step); the call rebuilds the work items.  Synchronises. */
+int bh_energy_grid(void* ctx, const bh_particles* p, double G, double eps2, double* out,
+                   void* stream);
 
 /* Learned cost model (MLP 8 -> 32 -> 32 -> 1, FP32, ReLU).
  * params: device float[BH_MLP_NPARAM] = W1[32x8], b1[32], W2[32x32], b2[32],

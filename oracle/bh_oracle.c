@@ -327,6 +327,75 @@ void bho_energy(int64_t n, const double *x, const double *y, const double *m,
     out2[1] = 0.5 * cs_val(&K);
 }
 
+/* Potential of the law evaluated by the grid method (no reference
+ * counterpart; it serves the energy diagnostic of large runs): for each
+ * target i, phi_i = sum_{j != i in the 3x3 cells} m_j log(r_ij^2 + eps2)
+ *                 + sum_{non-empty cells k outside} M_k log(|X_k - x_i|^2 + eps2),
+ * the decomposition of bho_forces with log(s + eps2) in place of
+ * (x_j - x_i) / (s + eps2).  U_grid = Gc / 4 * sum_i m_i phi_i.  Compensated
+ * sums per target. */
+void bho_potential(int G, const int64_t *offsets, const int32_t *key, int64_t n,
+                   const double *x, const double *y, const double *m, const double *com,
+                   double eps2, const int64_t *targets, int64_t ntargets, int nthreads,
+                   double *phi)
+{
+    int64_t ncells = (int64_t)G * G;
+    int64_t K = 0;
+    for (int64_t c = 0; c < ncells; ++c) K += (offsets[c + 1] > offsets[c]);
+    size_t kb = (size_t)(K > 0 ? K : 1);
+    int32_t *kx = (int32_t *)malloc(sizeof(int32_t) * kb);
+    int32_t *ky = (int32_t *)malloc(sizeof(int32_t) * kb);
+    double *cX = (double *)malloc(sizeof(double) * kb);
+    double *cY = (double *)malloc(sizeof(double) * kb);
+    double *cM = (double *)malloc(sizeof(double) * kb);
+    K = 0;
+    for (int64_t c = 0; c < ncells; ++c)
+        if (offsets[c + 1] > offsets[c]) {
+            kx[K] = (int32_t)(c / G);
+            ky[K] = (int32_t)(c % G);
+            cX[K] = com[3 * c];
+            cY[K] = com[3 * c + 1];
+            cM[K] = com[3 * c + 2];
+            ++K;
+        }
+    int64_t nt = targets ? ntargets : n;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+    for (int64_t t = 0; t < nt; ++t) {
+        int64_t i = targets ? targets[t] : t;
+        int cx = key[i] / G, cy = key[i] % G;
+        double xi = x[i], yi = y[i];
+        csum s = {0, 0};
+        for (int dx = -1; dx <= 1; ++dx) {
+            int nx = cx + dx;
+            if (nx < 0 || nx >= G) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int ny = cy + dy;
+                if (ny < 0 || ny >= G) continue;
+                int64_t cc = (int64_t)nx * G + ny;
+                for (int64_t j = offsets[cc]; j < offsets[cc + 1]; ++j) {
+                    if (j == i) continue;
+                    double wx = x[j] - xi, wy = y[j] - yi;
+                    cs_add(&s, m[j] * log(wx * wx + wy * wy + eps2));
+                }
+            }
+        }
+        for (int64_t k = 0; k < K; ++k) {
+            if (abs(kx[k] - cx) <= 1 && abs(ky[k] - cy) <= 1) continue;
+            double wx = cX[k] - xi, wy = cY[k] - yi;
+            cs_add(&s, cM[k] * log(wx * wx + wy * wy + eps2));
+        }
+        phi[t] = cs_val(&s);
+    }
+    free(kx);
+    free(ky);
+    free(cX);
+    free(cY);
+    free(cM);
+}
+
 /* Symplectic-Euler tick, gpu/grid.py:159-174 (g_tick_particles):
  * v += a*dt; a = 0; x += v*dt. Separate multiply and add (no FMA). */
 void bho_tick(int64_t n, double *x, double *y, double *vx, double *vy, double *ax, double *ay,

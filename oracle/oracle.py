@@ -20,6 +20,7 @@ References (python-bh/...):
                           barneshut/kernels/gravity/cpu_numba.py:30-92,
                           barneshut/kernels/grid_decomposition/gpu/grid.py:116-268
   direct sum              barneshut/implementations/base.py:16-32
+  grid potential          (no reference counterpart: the energy diagnostic of long runs)
   tick                    barneshut/kernels/grid_decomposition/gpu/grid.py:159-174
 """
 from __future__ import annotations
@@ -67,6 +68,9 @@ def lib():
         L.bho_interactions.argtypes = [ctypes.c_int, _i64p, _i64p]
         L.bho_energy.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_int, _dp]
+        L.bho_potential.argtypes = [ctypes.c_int, _i64p, _i32p, ctypes.c_int64, _dp, _dp, _dp,
+                                    _dp, ctypes.c_double, _i64p, ctypes.c_int64, ctypes.c_int,
+                                    _dp]
         _lib = L
     return _lib
 
@@ -217,6 +221,34 @@ def energy(x, y, m, vx, vy, Gc=1.0, eps2=0.0, nthreads=0):
     lib().bho_energy(len(x), _d(x), _d(y), _d(m), _d(vx), _d(vy), float(Gc), float(eps2),
                      int(nthreads), _d(out))
     return float(out[0]), float(out[1])
+
+
+def potential(G, offsets, key_sorted, x, y, m, com, eps2=0.0, targets=None, nthreads=0):
+    """Grid-method potential per target: phi_i = sum over the 3x3 cells' other
+    particles of m_j log(r^2 + eps2) plus sum over the other non-empty cells of
+    M_k log(|X_k - x_i|^2 + eps2).  U_grid = Gc / 4 * sum_i m_i phi_i.  No
+    reference counterpart (python-bh reports no energy)."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    key_sorted = np.ascontiguousarray(key_sorted, dtype=np.int32)
+    x, y, m, com = _f64c(x), _f64c(y), _f64c(m), _f64c(com)
+    n = len(x)
+    G = int(G)
+    if not len(y) == len(m) == len(key_sorted) == n or len(offsets) != G * G + 1 \
+            or com.size != 3 * G * G:
+        raise ValueError("potential: arrays must match n particles and G*G cells")
+    _check_offsets(offsets, n)
+    _check_keys(key_sorted, G * G)
+    if targets is not None:
+        targets = np.ascontiguousarray(targets, dtype=np.int64)
+        if len(targets) and (int(targets.min()) < 0 or int(targets.max()) >= n):
+            raise ValueError(f"targets must lie in [0, {n})")
+        nt, tp = len(targets), _i64(targets)
+    else:
+        nt, tp = n, None
+    phi = np.empty(nt)
+    lib().bho_potential(G, _i64(offsets), _i32(key_sorted), n, _d(x), _d(y), _d(m), _d(com),
+                        float(eps2), tp, nt, int(nthreads), _d(phi))
+    return phi
 
 
 def tick(x, y, vx, vy, ax, ay, dt):

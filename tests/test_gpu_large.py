@@ -58,6 +58,47 @@ def test_energy_matches_oracle():
         assert got[s, 1] == pytest.approx(K, rel=1e-13)
 
 
+def test_energy_grid_matches_oracle():
+    """bh_energy_grid (the potential by the grid method, FP32 cell-relative
+    logs) against the oracle's FP64 grid potential: one clustered system and
+    two batched systems (per system), and on 1x1 / 2x2 grids, where the grid
+    potential is the direct pair sum, against bh_energy's direct FP64 sum."""
+    from barneshut.engine import Engine
+    a = inputs.gaussian_mixture(20000, 6, (0.005, 0.1), seed=63)
+    b = inputs.uniform(7000, seed=64)
+    for d in (a, b):
+        d["vx"] = np.random.default_rng(3).normal(0, 0.1, len(d["px"]))
+        d["vy"] = np.random.default_rng(4).normal(0, 0.1, len(d["px"]))
+    both = {k: np.concatenate([a[k], b[k]]) for k in a}
+    eng = Engine([20000, 7000], [48, 24])
+    eng.load(both)
+    got = eng.energy_grid(1.3, 1e-6)
+    h = eng.fetch()
+    keys = eng.keys().astype(np.int64)
+    off = eng.offsets().astype(np.int64)
+    eng.close()
+    for s, (n, G, p0, c0) in enumerate(((20000, 48, 0, 0), (7000, 24, 20000, 48 * 48))):
+        sl = slice(p0, p0 + n)
+        o = off[c0:c0 + G * G + 1] - p0
+        com = orc.summarize(o, h["px"][sl], h["py"][sl], h["mass"][sl])
+        phi = orc.potential(G, o, keys[sl] - c0, h["px"][sl], h["py"][sl], h["mass"][sl], com,
+                            1e-6)
+        U = 0.25 * 1.3 * float(np.sum(h["mass"][sl] * phi))
+        K = 0.5 * float(np.sum(h["mass"][sl] * (h["vx"][sl] ** 2 + h["vy"][sl] ** 2)))
+        print(f"[energy_grid s={s}] U={got[s, 0]:.12e} oracle {U:.12e} "
+              f"rel {abs(got[s, 0] - U) / abs(U):.2e}", flush=True)
+        assert got[s, 0] == pytest.approx(U, rel=1e-6)
+        assert got[s, 1] == pytest.approx(K, rel=1e-13)
+    for G in (1, 2):
+        e = Engine([6000], [G])
+        e.load(inputs.gaussian_mixture(6000, 3, (0.02, 0.2), seed=65 + G))
+        Ug = e.energy_grid(0.8, 1e-6)[0, 0]
+        Ud = e.energy(0.8, 1e-6)[0, 0]
+        e.close()
+        print(f"[energy_grid G={G}] grid {Ug:.12e} direct {Ud:.12e}", flush=True)
+        assert Ug == pytest.approx(Ud, rel=1e-6)
+
+
 def _trajectory(d, G, precision, steps, model=None, g=GC):
     import torch
     eng = _engine(len(d["px"]), G)
@@ -139,6 +180,10 @@ def _sorted_under_total_order(h, keys):
 
 
 def test_large_full_size_one_step():
+    """One step of the 67M-particle workload under the bench's softened law
+    (eps = 1e-5).  Softening is needed here: clipping the mixture to the box
+    stacks 3,408 particles exactly on the four corners, where the unsoftened
+    law is 0/0.  Every acceleration must be finite."""
     import torch
     from barneshut import costmodel
     n, G = 67108864, 4096
@@ -159,11 +204,11 @@ def test_large_full_size_one_step():
     eng.ctx.build_items()
     eng.ctx.learned_order(model.params, order)
     ev[1].record()
-    eng.ctx.force_pass(eng.parts, GC, 0.0, 0.0, "fp32", order)
+    eng.ctx.force_pass(eng.parts, GC, BENCH_EPS2, 0.0, "fp32", order)
     ev[2].record()
     torch.cuda.synchronize()
     print(f"[large] FP32 force pass done: {ev[1].elapsed_time(ev[2]):.1f} ms", flush=True)
-    eng.ctx.recompute(eng.parts, GC, 0.0, 0.0)
+    eng.ctx.recompute(eng.parts, GC, BENCH_EPS2, 0.0)
     ev[3].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -203,7 +248,9 @@ def test_large_full_size_one_step():
     # FP32 accelerations of a random sample against the oracle
     idx = np.sort(np.random.default_rng(5).choice(n, 1024, replace=False))
     t = time.time()
-    rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, 0.0, 0.0, targets=idx)
+    assert np.isfinite(h["ax"]).all() and np.isfinite(h["ay"]).all()
+    rx, ry = orc.forces(G, off, ks, h["px"], h["py"], h["mass"], ref, GC, BENCH_EPS2, 0.0,
+                        targets=idx)
     num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
     den = np.hypot(rx, ry)
     norm = float(np.sqrt((num ** 2).sum() / (den ** 2).sum()))

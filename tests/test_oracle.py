@@ -308,6 +308,62 @@ def test_energy_gradient_is_the_force_law():
             assert -grad == pytest.approx(m[i] * acc[i], rel=1e-6, abs=1e-8)
 
 
+def _grid_energy(o, G, Gc, eps2):
+    phi = orc.potential(G, o["offsets"], o["key"], o["px"], o["py"], o["m"], o["com"], eps2)
+    return 0.25 * Gc * float(np.sum(o["m"] * phi)), phi
+
+
+@pytest.mark.parametrize("G", [1, 2])
+def test_grid_potential_equals_direct_energy_on_tiny_grids(G):
+    """G <= 2: every cell is a neighbour, so the grid potential is the direct
+    pair sum and U_grid equals the direct energy."""
+    d = inputs.gaussian_mixture(2500, 3, (0.02, 0.2), seed=30 + G)
+    o = _method(d, G, 1.0, 1e-6, 0.0)
+    U, _ = _grid_energy(o, G, 1.7, 1e-6)
+    Ud, _ = orc.energy(o["px"], o["py"], o["m"], o["vx"], o["vy"], 1.7, 1e-6)
+    assert U == pytest.approx(Ud, rel=1e-13)
+
+
+def test_grid_potential_single_particle_cells_equals_direct_energy():
+    """Every non-empty cell holds one particle: each far COM is the particle
+    itself, so the far field is exact and U_grid equals the direct energy.
+    On a dense grid (several particles per far cell) it does not: the
+    monopole far field is an approximation, guarding against a far field that
+    sums particles."""
+    G = 40
+    rng = np.random.default_rng(31)
+    cells = rng.choice(np.arange(1, G * G - 1), 300, replace=False)
+    x = (cells // G + rng.uniform(0.2, 0.8, 300)) / G
+    y = (cells % G + rng.uniform(0.2, 0.8, 300)) / G
+    x[:2] = [0.0, 1.0]
+    y[:2] = [0.0, 1.0]
+    d = inputs._pack(x, y, rng.uniform(1, 2, 300))
+    o = _method(d, G)
+    assert o["counts"].max() == 1
+    U, _ = _grid_energy(o, G, 1.0, 0.0)
+    Ud, _ = orc.energy(o["px"], o["py"], o["m"], o["vx"], o["vy"], 1.0, 0.0)
+    assert U == pytest.approx(Ud, rel=1e-13)
+    d2 = inputs.uniform(3000, seed=32)
+    o2 = _method(d2, 16)
+    U2, _ = _grid_energy(o2, 16, 1.0, 0.0)
+    Ud2, _ = orc.energy(o2["px"], o2["py"], o2["m"], o2["vx"], o2["vy"], 1.0, 0.0)
+    assert 1e-9 < abs(U2 - Ud2) / abs(Ud2) < 1e-2
+
+
+def test_grid_potential_two_body_closed_form():
+    """Two particles far apart on a 16 x 16 grid (each alone in its cell, the
+    other a far COM): phi_0 = m_1 log(r^2 + eps2), phi_1 = m_0 log(r^2 + eps2)."""
+    x, y = np.array([0.05, 0.93]), np.array([0.1, 0.77])
+    d = inputs._pack(x, y, np.array([1.5, 2.5]))
+    o = _method(d, 16)
+    _, phi = _grid_energy(o, 16, 1.0, 1e-4)
+    r2 = (0.93 - 0.05) ** 2 + (0.77 - 0.1) ** 2
+    m_of = dict(zip(o["pid"].tolist(), o["m"].tolist()))
+    for k in range(2):
+        other = m_of[1 - int(o["pid"][k])]
+        assert phi[k] == pytest.approx(other * np.log(r2 + 1e-4), rel=1e-14)
+
+
 def test_oracle_refuses_inconsistent_inputs():
     """The C restatement indexes per-cell arrays by key and offset: its
     wrappers refuse keys, offsets, shapes and targets that would run out of

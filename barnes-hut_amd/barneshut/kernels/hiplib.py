@@ -37,7 +37,7 @@ EXPORTED = (
     "bh_force_pass", "bh_tick_particles", "bh_step", "bh_step_graph", "bh_step_learned", "bh_read_stats", "bh_export",
     "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy", "bh_set_recompute",
-    "bh_recompute", "bh_step_noalloc", "bh_energy_grid",
+    "bh_recompute", "bh_step_noalloc", "bh_energy_grid", "bh_force_pass_part",
 )
 # limits of the allocation-free step (bh_step_noalloc)
 NOALLOC_MAX_N = 262144
@@ -101,6 +101,8 @@ def load(path: str = LIB_PATH):
         "bh_build_items": (ctypes.c_int, [vp, vp]),
         "bh_evaluate_boxes": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), i32, vp, vp]),
         "bh_force_pass": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), i32, vp, vp]),
+        "bh_force_pass_part": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), i32, vp, i32, i32,
+                                              vp]),
         "bh_tick_particles": (ctypes.c_int, [vp, P(BhParticles), dbl, vp]),
         "bh_step": (ctypes.c_int, [vp, P(BhParticles), P(BhParticles), P(BhLaw), i32, dbl, i32,
                                    vp]),
@@ -301,13 +303,21 @@ class Context:
                                        ptr(order), stream_handle(stream)))
 
     def force_pass(self, parts, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", order=None,
-                   stream=None):
+                   stream=None, part=None, nparts=1):
+        """The force kernels on the current items.  part / nparts: one of
+        nparts launches covering contiguous ranges of the work queue (call
+        part = 0 .. nparts-1 in order; the result equals one launch)."""
         p = self._parts(parts)
         self._order(order)
         law = BhLaw(float(G), float(eps2), float(r2_cut))
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        check(self.L.bh_force_pass(self.h, ctypes.byref(p), ctypes.byref(law), prec, ptr(order),
-                                   stream_handle(stream)))
+        if part is None:
+            check(self.L.bh_force_pass(self.h, ctypes.byref(p), ctypes.byref(law), prec,
+                                       ptr(order), stream_handle(stream)))
+        else:
+            check(self.L.bh_force_pass_part(self.h, ctypes.byref(p), ctypes.byref(law), prec,
+                                            ptr(order), int(part), int(nparts),
+                                            stream_handle(stream)))
 
     def tick(self, parts, dt, stream=None):
         p = self._parts(parts)

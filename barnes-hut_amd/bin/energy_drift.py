@@ -49,6 +49,8 @@ def main(argv=None):
     ap.add_argument("--energy-every", type=int, default=1)
     ap.add_argument("--budget-s", type=float, default=0.0)
     ap.add_argument("--direct", action="store_true")
+    ap.add_argument("--parts", type=int, default=4,
+                    help="force pass in this many launches (progress between them)")
     a = ap.parse_args(argv)
 
     import torch
@@ -108,9 +110,11 @@ def main(argv=None):
             eng.ctx.learned_order(model.params, order)
         eng.check()
         log(f"step {s}: placed")
-        eng.ctx.force_pass(eng.parts, a.G, a.eps2, 0.0, a.precision, order)
-        eng.check()
-        log(f"step {s}: forces")
+        for part in range(a.parts):   # launches over ranges of the work queue
+            eng.ctx.force_pass(eng.parts, a.G, a.eps2, 0.0, a.precision, order, part=part,
+                               nparts=a.parts)
+            eng.check()
+            log(f"step {s}: forces {part + 1}/{a.parts}")
         eng.timestep(a.dt)
         eng.check()
         t_step = time.perf_counter() - t0

@@ -616,8 +616,16 @@ int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32
 int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
                   const int32_t* order, void* stream)
 {
+    return bh_force_pass_part(ctx, p, law, precision, order, 0, 1, stream);
+}
+
+int bh_force_pass_part(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
+                       const int32_t* order, int32_t part, int32_t nparts, void* stream)
+{
     Ctx* c = (Ctx*)ctx;
     CTX_READY(c);
+    if (nparts < 1 || nparts > 4096 || part < 0 || part >= nparts)
+        return fail(BH_EINVAL, "need 0 <= part < nparts <= 4096");
     int r = check_parts(p, "particles");
     if (r) return r;
     if (!law) return fail(BH_EINVAL, "law is NULL");
@@ -657,16 +665,24 @@ int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t p
     fa.part_n = c->N;
     const bool rc = fp32v && c->rc_kappa > 0.0;
     fa.rc = rc ? 1 : 0;
+    fa.qpart = part;
+    fa.nqparts = nparts;
     if (rc) {   // the FP32 result goes to scratch; the recompute adds it (or the FP64 one)
-        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(2 * c->N, 256, 8192)), dim3(256), 0, st,
-                           c->rc_a32, 2 * c->N, 0.0);
+        if (part == 0)
+            hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(2 * c->N, 256, 8192)), dim3(256), 0, st,
+                               c->rc_a32, 2 * c->N, 0.0);
         fa.ax = c->rc_a32;
         fa.ay = c->rc_a32 + c->N;
     }
+    if (part == 0) c->rc_ready = 0;   // until the last part: no recompute of a partial pass
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
     void* args[] = {&fa};
     unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
     HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
+    if (part < nparts - 1) {   // the slice combine and the recompute follow the last part
+        LAUNCH_CHECK("evaluate");
+        return BH_OK;
+    }
     if (fa.split > 1) {
         const int pslots = 64 * c->tune.p;
         hipLaunchKernelGGL(k_force_combine, dim3(grid_for((int64_t)fa.n_tail * pslots, 256,

@@ -142,6 +142,15 @@ __device__ __forceinline__ int next_item(int32_t* stats, int lane)
     return uniform_int(q);
 }
 
+// Queue positions [begin, end) of launch `part` of `nparts`: contiguous
+// ranges of the `total` positions (the launches of one pass run one after
+// the other on one stream; each particle's sums do not depend on the split).
+__device__ __forceinline__ void part_range(int total, int part, int nparts, int& begin, int& end)
+{
+    begin = (int)(((int64_t)total * part) / nparts);
+    end = (int)(((int64_t)total * (part + 1)) / nparts);
+}
+
 __device__ __forceinline__ bool meets(int b0, int b1, int lo, int hi)
 {
     return lo < hi && b0 < hi && b1 > lo;
@@ -202,9 +211,11 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
     __shared__ NearE ntile[BH_WAVE];
     const int lane = threadIdx.x;
     const int nitems = A.stats[ST_NITEMS];
+    int qb, qe;
+    part_range(nitems, A.qpart, A.nqparts, qb, qe);
     for (;;) {
-        const int q = next_item(A.stats, lane);
-        if (q >= nitems) break;
+        const int q = qb + next_item(A.stats, lane);
+        if (q >= qe) break;
         const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
         const int itx = A.order ? uniform_int(A.order[q]) : q;
         if ((unsigned)itx >= (unsigned)nitems) continue;   // malformed order entry
@@ -450,9 +461,11 @@ __global__ __launch_bounds__(64, (F32Waves<P, KAHAN>::value)) void k_force32(For
     const int nt = S > 1 ? min(A.n_tail, nitems) : 0;
     const int nu = nitems - nt;
     const int total = nu + nt * S;
+    int qb, qe;
+    part_range(total, A.qpart, A.nqparts, qb, qe);
     for (;;) {
-        const int q = next_item(A.stats, lane);
-        if (q >= total) break;
+        const int q = qb + next_item(A.stats, lane);
+        if (q >= qe) break;
         int pos = q, part = -1;
         if (q >= nu) {   // slice-major: every tail item's slice 0, then slice 1, ...
             const int r = q - nu;
@@ -792,9 +805,11 @@ __global__ __launch_bounds__(64) void k_force32m(ForceArgs A)
     const int lane = threadIdx.x;
     const int bc = lane & 3;
     const int nitems = A.stats[ST_NITEMS];
+    int qb, qe;
+    part_range(nitems, A.qpart, A.nqparts, qb, qe);
     for (;;) {
-        const int q = next_item(A.stats, lane);
-        if (q >= nitems) break;
+        const int q = qb + next_item(A.stats, lane);
+        if (q >= qe) break;
         const long long t0 = A.cost_log ? (long long)wall_clock64() : 0;
         const int itx = A.order ? uniform_int(A.order[q]) : q;
         if ((unsigned)itx >= (unsigned)nitems) continue;

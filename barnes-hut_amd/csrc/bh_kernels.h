@@ -35,6 +35,9 @@ struct ForceArgs {
     int split, n_tail;
     double* part;
     int64_t part_n;
+    // this launch covers queue positions [total * qpart / nqparts,
+    // total * (qpart + 1) / nqparts) (one launch: part 0 of 1)
+    int qpart, nqparts;
     // FP64 recompute on (FP32 kernels): ax / ay point at zeroed scratch
     // planes of part_n doubles, and ay[part_n + i] receives particle i's sum
     // over its near-field terms of (m_j / r_ij)^2 (an upper bound of sum

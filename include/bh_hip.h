@@ -129,6 +129,14 @@ int bh_evaluate_boxes(void* ctx, const bh_particles* p, const bh_law* law, int32
  * order is bh_build_items followed by this call. */
 int bh_force_pass(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
                   const int32_t* order, void* stream);
+/* bh_force_pass in nparts launches: call it for part = 0 .. nparts-1 in
+ * order (same arguments otherwise); part k runs the k-th contiguous range of
+ * the work queue, the last part adds the slices and runs the recompute.  The
+ * result is bit-identical to bh_force_pass (a particle's sums do not depend
+ * on which launch runs its pieces).  For callers that need to return to the
+ * host during a long pass (progress, deadlines).  No reference counterpart. */
+int bh_force_pass_part(void* ctx, const bh_particles* p, const bh_law* law, int32_t precision,
+                       const int32_t* order, int32_t part, int32_t nparts, void* stream);
 int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream);
 
 /* FP64 recompute after an FP32 pass (precisions 1-3).  No reference

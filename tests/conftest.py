@@ -20,3 +20,13 @@ def gpu_available():
         return torch.cuda.is_available()
     except Exception:
         return False
+
+
+@pytest.fixture
+def progress(capsys):
+    """print() past pytest's output capture: long GPU tests report each phase
+    as it ends, so a run that is merely long never looks silent."""
+    def _progress(msg):
+        with capsys.disabled():
+            print(msg, flush=True)
+    return _progress

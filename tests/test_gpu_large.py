@@ -179,17 +179,19 @@ def _sorted_under_total_order(h, keys):
     return bool(ok.all())
 
 
-def test_large_full_size_one_step():
+def test_large_full_size_one_step(progress):
     """One step of the 67M-particle workload under the bench's softened law
     (eps = 1e-5).  Softening is needed here: clipping the mixture to the box
     stacks 3,408 particles exactly on the four corners, where the unsoftened
-    law is 0/0.  Every acceleration must be finite."""
+    law is 0/0.  Every acceleration must be finite.  The force pass runs in
+    8 launches (bh_force_pass_part) so the test reports progress every ~20 s
+    of its ~3 minutes."""
     import torch
     from barneshut import costmodel
     n, G = 67108864, 4096
     t = time.time()
     d = inputs.gaussian_mixture(n, 4096, SIGMA, seed=81, background=0.2)
-    print(f"[large] generated N={n} in {time.time() - t:.1f}s", flush=True)
+    progress(f"[large] generated N={n} in {time.time() - t:.1f}s")
     eng = _engine(n, G)
     eng.load(d)
     model = costmodel.CostModel.load(MODEL, device=torch.device("cuda"))
@@ -204,10 +206,15 @@ def test_large_full_size_one_step():
     eng.ctx.build_items()
     eng.ctx.learned_order(model.params, order)
     ev[1].record()
-    eng.ctx.force_pass(eng.parts, GC, BENCH_EPS2, 0.0, "fp32", order)
+    nparts = 8
+    for part in range(nparts):
+        eng.ctx.force_pass(eng.parts, GC, BENCH_EPS2, 0.0, "fp32", order, part=part,
+                           nparts=nparts)
+        torch.cuda.synchronize()
+        progress(f"[large] force pass part {part + 1}/{nparts} done")
     ev[2].record()
     torch.cuda.synchronize()
-    print(f"[large] FP32 force pass done: {ev[1].elapsed_time(ev[2]):.1f} ms", flush=True)
+    progress(f"[large] FP32 force pass done: {ev[1].elapsed_time(ev[2]):.1f} ms")
     eng.ctx.recompute(eng.parts, GC, BENCH_EPS2, 0.0)
     ev[3].record()
     torch.cuda.synchronize()
@@ -217,10 +224,10 @@ def test_large_full_size_one_step():
     inter = eng.interactions()[-1]
     fms = ev[1].elapsed_time(ev[2])
     tflops = 11.0 * float(inter.sum()) / (fms * 1e-3) / 1e12
-    print(f"[large] tree+summary+items+order {ev[0].elapsed_time(ev[1]):.1f} ms, "
+    progress(f"[large] tree+summary+items+order {ev[0].elapsed_time(ev[1]):.1f} ms, "
           f"force {fms:.1f} ms, near {inter[0]:.3e} far {inter[1]:.3e}, {tflops:.1f} TFLOP/s, "
           f"FP64 recompute {ev[2].elapsed_time(ev[3]):.1f} ms for {nrc} particles, "
-          f"{n / wall / 1e6:.3f} Mparticle-steps/s (one step, no tick)", flush=True)
+          f"{n / wall / 1e6:.3f} Mparticle-steps/s (one step, no tick)")
 
     # placement: keys bit-exact, counts, sorted rows that permute the input
     bb = orc.bbox(d["px"], d["py"], G)
@@ -235,8 +242,9 @@ def test_large_full_size_one_step():
     assert np.array_equal(np.sort(pid), np.arange(n))
     for k in ("px", "py", "mass"):
         assert np.array_equal(h[k], d[k][pid])
+    progress("[large] fetched")
     assert _sorted_under_total_order(h, ks)
-    print("[large] placement checks passed", flush=True)
+    progress("[large] placement checks passed")
 
     # COMs against the oracle
     off = eng.offsets().astype(np.int64)
@@ -254,8 +262,8 @@ def test_large_full_size_one_step():
     num = np.hypot(h["ax"][idx] - rx, h["ay"][idx] - ry)
     den = np.hypot(rx, ry)
     norm = float(np.sqrt((num ** 2).sum() / (den ** 2).sum()))
-    print(f"[large] oracle sample ({time.time() - t:.1f}s): normwise {norm:.3e}, "
-          f"p99 {np.quantile(num / den, 0.99):.3e}, max {(num / den).max():.3e}", flush=True)
+    progress(f"[large] oracle sample ({time.time() - t:.1f}s): normwise {norm:.3e}, "
+          f"p99 {np.quantile(num / den, 0.99):.3e}, max {(num / den).max():.3e}")
     assert norm <= 1e-5
     assert (num / den).max() <= 1e-5
     eng.close()

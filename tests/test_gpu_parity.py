@@ -366,6 +366,36 @@ def test_tick_bit_exact():
     eng.close()
 
 
+PART_CASES = [("fp32", {"p": 2, "yspan": 64, "waves_per_cu": 8, "tail_split": 4,
+                        "tail_items": 1 << 20}),
+              ("fp64", {"p": 2, "yspan": 512, "waves_per_cu": 8}),
+              ("fp32_mfma", {"p": 1, "yspan": 16, "waves_per_cu": 8})]
+
+
+@pytest.mark.parametrize("prec,tun", PART_CASES, ids=[c[0] for c in PART_CASES])
+def test_force_pass_in_parts_equals_one_launch(prec, tun):
+    """bh_force_pass_part: the pass as 7 launches over contiguous ranges of the
+    work queue (far-field slices of split items in different launches, the
+    recompute after the last) gives the accelerations of one launch bit for
+    bit, and the same recompute count."""
+    d = inputs.gaussian_mixture(50000, 8, (0.005, 0.05), seed=19)
+    eng = _forces_setup(d, 64, tun)
+    eng.ctx.build_items()
+    res = []
+    for nparts in (1, 7):
+        eng.parts["ax"].zero_()
+        eng.parts["ay"].zero_()
+        for part in range(nparts):
+            eng.ctx.force_pass(eng.parts, 1e-6, 1e-10, 0.0, prec, part=part, nparts=nparts)
+        st = eng.ctx.stats()
+        g = eng.fetch()
+        res.append((g["ax"], g["ay"], int(st[6])))
+    np.testing.assert_array_equal(res[1][0], res[0][0])
+    np.testing.assert_array_equal(res[1][1], res[0][1])
+    assert res[1][2] == res[0][2]
+    eng.close()
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp64_ieee"])
 def test_fp64_forces_independent_of_tuning(prec):
     """FP64 sums run in a fixed order per particle: bit-identical across

@@ -25,7 +25,7 @@ class BatchedSystems:
 
     def __init__(self, sizes, grid_dims, data, law=(1.0, 0.0, 0.0), precision="fp32", dt=0.0,
                  tuning=None, device=0, model=None, graph=True, autotune_candidates=None,
-                 recompute_kappa=2.0):
+                 recompute_kappa=1.5):
         """tuning: None (BATCHED_TUNING), a dict, or "auto": the autotuner picks
         the launch shape on the loaded state (autotune_candidates narrows it).
         recompute_kappa: FP32 precisions re-evaluate in FP64 the particles whose

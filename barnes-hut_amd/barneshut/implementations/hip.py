@@ -29,7 +29,7 @@ Extra keys, section [hip]:
               2, 512, 32: the fastest on the 4M-particle workloads; FP64 items
               are never split, so no slice planes are allocated)
   recompute_kappa  FP32 precisions: FP64 recompute of the particles whose
-              rounding-error bound exceeds kappa * |a| (default 2; 0 = off)
+              rounding-error bound exceeds kappa * |a| (default 1.5; 0 = off)
 """
 import numpy as np
 
@@ -61,7 +61,7 @@ class HipBarnesHut(BaseBarnesHut):
                                                            fallback="32")),
                                tail_split=1, tail_items=0)
         self.device = int(Config.get("hip", "device", fallback="0"))
-        self.recompute_kappa = float(Config.get("hip", "recompute_kappa", fallback="2.0"))
+        self.recompute_kappa = float(Config.get("hip", "recompute_kappa", fallback="1.5"))
         self.engine = None
         self.model = None
         self.order = None

@@ -255,7 +255,7 @@ class Context:
     def set_cost_log(self, on: bool):
         check(self.L.bh_set_cost_log(self.h, 1 if on else 0))
 
-    def set_recompute(self, kappa=2.0, deferred=False):
+    def set_recompute(self, kappa=1.5, deferred=False):
         """FP64 recompute after FP32 passes of the particles whose rounding-
         error bound sqrt(sum |term|^2) exceeds kappa * |a| (kappa <= 0: off).
         deferred: force_pass stops after the FP32 kernels and the caller runs
@@ -414,7 +414,7 @@ class Context:
 
 
 def step_noalloc(a, b, grid_dim, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", dt=0.0,
-                 do_tick=True, tuning=None, kappa=2.0, stats=None, stream=None):
+                 do_tick=True, tuning=None, kappa=1.5, stats=None, stream=None):
     """bh_step_noalloc: one full round of one system (a placed and sorted
     into b, then summarize, forces, tick on b) with no device allocation; all
     scratch comes from the library's static per-GPU array.  tuning: dict of

@@ -88,7 +88,7 @@ struct bh_ctx_s {
     double* energy_buf = nullptr;   // 2 x N + S, allocated by the first energy call
     // FP64 recompute: threshold, the FP32 pass's per-particle scratch, the
     // flagged list and groups, per-block sums of M^2
-    double rc_kappa = 2.0;
+    double rc_kappa = 1.5;
     int rc_deferred = 0;
     int rc_ready = 0;        // the last force pass prepared a32 / s2n
     double* rc_a32 = nullptr;      // the FP32 pass: x plane, y plane, near-field sums

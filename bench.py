@@ -27,7 +27,7 @@ per-cell COM, forces, tick.  Paths on the same input:
   * the same FP32 path with the learned work-queue order (the cost model in
     barnes-hut_amd/costmodels/clusters256.npz; --cost-model none disables).
 The FP32 paths include the FP64 recompute of the particles whose FP32 terms
-nearly cancel (kappa 2, --recompute-kappa 0 disables); it is timed apart from
+nearly cancel (kappa 1.5, --recompute-kappa 0 disables); it is timed apart from
 the force kernel.  `value` is the FP32 + learned-order path's whole-job
 throughput.
 
@@ -74,7 +74,7 @@ FLOP_PER_INTERACTION = 11
 # 1 GiB device copy (FETCH x2.0, WRITE x1.0), measured on the first two
 # steps of this workload with the default launch shape
 # (profiles/force32_gauss64_n4M_g1024_hbm_traffic.md).
-TRAFFIC = 5.10e9
+TRAFFIC = 5.09e9
 # Issue-limited ceiling of the FP32 far-field term: 6 VALU instructions per
 # interaction (v_rcp_f32 co-issued), no packed FP32, 256 CUs x 4 SIMDs x 16
 # lanes x 2.4 GHz = 39.3e12 lane-instructions/s -> 6.55e12 interactions/s.
@@ -84,7 +84,7 @@ ISSUE_CEILING_TFLOPS = LANE_OPS / 6 * FLOP_PER_INTERACTION / 1e12
 # for the refined quotient, m * r0, 2 fma into the sums) + v_rcp_f64
 ISSUE_CEILING64_TFLOPS = LANE_OPS / 10 * FLOP_PER_INTERACTION / 1e12
 # FP64 recompute threshold of the FP32 paths (--recompute-kappa)
-RECOMPUTE_KAPPA = 2.0
+RECOMPUTE_KAPPA = 1.5
 
 
 def log(msg):

@@ -153,7 +153,7 @@ int bh_tick_particles(void* ctx, const bh_particles* p, double dt, void* stream)
  * bh_evaluate_boxes and the step calls run it; deferred = 1: bh_force_pass
  * stops after the FP32 kernels and the caller runs bh_recompute before
  * anything else touches the particles (the step calls still run it).
- * Default: kappa 2, deferred 0. */
+ * Default: kappa 1.5, deferred 0. */
 int bh_set_recompute(void* ctx, double kappa, int32_t deferred);
 /* The recompute stage alone, for the last FP32 bh_force_pass on p (same law);
  * BH_ESTATE if that pass did not prepare it. */

@@ -147,7 +147,7 @@ def test_tiny_kappa_recomputes_everything(case):
 
 def test_per_particle_error_meets_target_on_clusters():
     """A clustered system on a fine grid: plain FP32 has particles over 1e-5
-    relative error; with the recompute (default kappa 2) none has."""
+    relative error; with the recompute (kappa 2) none has."""
     n, G, law = 1 << 18, 256, (1e-6, 0.0, 0.0)
     d = inputs.gaussian_mixture(n, 16, (0.001, 0.03), seed=305, background=0.2)
     eng = _placed(d, [n], [G])

@@ -235,8 +235,12 @@ __global__ __launch_bounds__(64) void k_force64(ForceArgs A)
             sig = ldexp(1.0, 1 - e2);
         }
         const double eps2 = A.eps2 * sig * sig, r2cut = A.r2cut * sig * sig;
-        // past the list: zero mass far away (the term is exactly 0)
-        const double pad = FAST ? 1e30 : 1e150;
+        // past the list: zero mass far away (the term is exactly 0).  In units of
+        // 1 / sig a padding q is ~2e120, so a pair of two padding entries keeps
+        // Q ~ 4e240 finite (an infinite Q would give R = NaN through
+        // fma(-inf, 0, 1)), and particles up to 1e59 cells from the origin stay
+        // far from it
+        const double pad = FAST ? 1e60 : 1e150;
 
         double xi[P], yi[P], sx[P], sy[P];
         int cy[P], gi[P];

@@ -80,9 +80,12 @@ TRAFFIC = 5.09e9
 # lanes x 2.4 GHz = 39.3e12 lane-instructions/s -> 6.55e12 interactions/s.
 LANE_OPS = 256 * 4 * 16 * 2.4e9
 ISSUE_CEILING_TFLOPS = LANE_OPS / 6 * FLOP_PER_INTERACTION / 1e12
-# FP64 far-field term: 10 VALU instructions (2 differences, 2 fma for r^2, 3
-# for the refined quotient, m * r0, 2 fma into the sums) + v_rcp_f64
-ISSUE_CEILING64_TFLOPS = LANE_OPS / 10 * FLOP_PER_INTERACTION / 1e12
+# FP64 far-field term, two entries per reciprocal: 10.5 VALU instructions
+# (2 differences, 2 fma for r^2, half of Q = q0 q1 and of its 3-instruction
+# refinement, 2 products for the weight, 2 fma into the sums) + half a
+# v_rcp_f64, which occupies 4 issue slots: 12.5 slots per term
+# (profiles/force64_gauss64_n4M_g1024_valu_issue.md)
+ISSUE_CEILING64_TFLOPS = LANE_OPS / 12.5 * FLOP_PER_INTERACTION / 1e12
 # FP64 recompute threshold of the FP32 paths (--recompute-kappa)
 RECOMPUTE_KAPPA = 1.5
 

@@ -38,6 +38,7 @@ EXPORTED = (
     "bh_interactions", "bh_interactions_async", "bh_item_features", "bh_mlp_forward", "bh_mlp_backward", "bh_mlp_adam",
     "bh_order_by_cost", "bh_learned_order", "bh_query", "bh_energy", "bh_set_recompute",
     "bh_recompute", "bh_step_noalloc", "bh_energy_grid", "bh_force_pass_part",
+    "bh_cloud_gravity",
 )
 # limits of the allocation-free step (bh_step_noalloc)
 NOALLOC_MAX_N = 262144
@@ -58,6 +59,10 @@ class BhParticles(ctypes.Structure):
 
 class BhLaw(ctypes.Structure):
     _fields_ = [("G", ctypes.c_double), ("eps2", ctypes.c_double), ("r2_cut", ctypes.c_double)]
+
+
+class BhCloud(ctypes.Structure):
+    _fields_ = [("rows", ctypes.c_void_p), ("n", ctypes.c_int64), ("ld", ctypes.c_int64)]
 
 
 class BhTuning(ctypes.Structure):
@@ -127,6 +132,7 @@ def load(path: str = LIB_PATH):
         "bh_recompute": (ctypes.c_int, [vp, P(BhParticles), P(BhLaw), vp]),
         "bh_step_noalloc": (ctypes.c_int, [P(BhParticles), P(BhParticles), i64, i32, P(BhLaw),
                                            i32, dbl, i32, P(BhTuning), dbl, vp, vp]),
+        "bh_cloud_gravity": (ctypes.c_int, [P(BhCloud), P(BhCloud), i32, P(BhLaw), i32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -434,6 +440,39 @@ def step_noalloc(a, b, grid_dim, G=1.0, eps2=0.0, r2_cut=0.0, precision="fp64", 
     check(L.bh_step_noalloc(ctypes.byref(pa), ctypes.byref(pb), int(n), int(grid_dim),
                             ctypes.byref(law), prec, float(dt), 1 if do_tick else 0, tun,
                             float(kappa), ptr(stats), stream_handle(stream)))
+
+
+def cloud_struct(rows, name):
+    """BhCloud over a (n, ld >= 7) float64 device tensor of particle rows
+    (columns px, py, mass, vx, vy, ax, ay, ...) with unit column stride."""
+    torch = _torch()
+    if not isinstance(rows, torch.Tensor) or rows.dtype != torch.float64 or not rows.is_cuda:
+        raise ValueError(f"{name} must be a float64 device tensor")
+    if rows.dim() != 2 or rows.shape[1] < 7:
+        raise ValueError(f"{name} must have shape (n, >= 7), got {tuple(rows.shape)}")
+    n = rows.shape[0]
+    if n and (rows.stride(1) != 1 or (n > 1 and rows.stride(0) < rows.shape[1])):
+        raise ValueError(f"{name} needs unit column stride and non-overlapping rows")
+    ld = rows.stride(0) if n > 1 else rows.shape[1]
+    return BhCloud(rows.data_ptr() if n else None, n, ld)
+
+
+def cloud_gravity(self_rows, other_rows=None, G=1.0, eps2=0.0, r2_cut=1e-5,
+                  update_other=False, stream=None):
+    """bh_cloud_gravity on device row tensors, in place: adds to self_rows'
+    ax, ay columns the attraction of other_rows (other_rows None: of the other
+    rows of self_rows), and with update_other also to other_rows'.  Default law:
+    the reference's CPU kernel (r^2 <= 1e-5 dropped, no softening)."""
+    same = other_rows is None
+    a = cloud_struct(self_rows, "self_rows")
+    b = a if same else cloud_struct(other_rows, "other_rows")
+    if not same and self_rows.device != other_rows.device:
+        raise ValueError("self_rows and other_rows must be on one device")
+    law = BhLaw(float(G), float(eps2), float(r2_cut))
+    L = load()
+    check(L.bh_cloud_gravity(ctypes.byref(a), None if same else ctypes.byref(b), int(same),
+                             ctypes.byref(law), 1 if update_other else 0,
+                             stream_handle(stream)))
 
 
 def _mlp_rows(x, name="x"):

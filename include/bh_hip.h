@@ -238,6 +238,30 @@ int bh_learned_order(void* ctx, const float* params, int32_t* order, void* strea
  * 4 work-item capacity, 5 compute units. */
 int bh_query(void* ctx, int32_t what, int64_t* out);
 
+/* Cloud-to-cloud direct gravity: the reference's gravity plugin
+ * fn(self_cloud, other_cloud, G, update_other)
+ * (python-bh/barneshut/kernels/gravity/cpu_numba.py:8-92; registry
+ * gravity/__init__.py:6-41; caller Cloud.apply_force, internals/cloud.py:113-114).
+ * A cloud is n rows of ld >= 7 doubles in device memory, in the reference's
+ * particle row layout (px, py, mass, vx, vy, ax, ay, ...:
+ * internals/particle.py:3-15). */
+typedef struct bh_cloud {
+    double* rows;
+    int64_t n;
+    int64_t ld;
+} bh_cloud;
+/* Adds to every self row's ax, ay: G * sum_j m_j (x_j - x_i) w(r2) over the
+ * other rows j, with law's w (G is law->G; the reference's CPU kernels use
+ * r2_cut = 1e-5, eps2 = 0).  same != 0: self and other are one cloud (other
+ * may be NULL), every j != i, and update_other is ignored
+ * (cpu_numba.py:9-13,30-47).  Otherwise, update_other != 0 also adds to every
+ * other row's ax, ay the sum over the self rows (cpu_numba.py:22-25,49-70);
+ * 0 updates self only (:26-28,72-92).  Both sums are taken from the positions
+ * the call starts with.  An empty cloud is a no-op.  Asynchronous on stream;
+ * the result of a call depends only on its inputs and the two sizes. */
+int bh_cloud_gravity(const bh_cloud* self, const bh_cloud* other, int32_t same,
+                     const bh_law* law, int32_t update_other, void* stream);
+
 /* Allocation-free step.  No counterpart in the reference: it is for callers
  * that cannot allocate device memory -- code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc.  bh_step for ONE system of

@@ -396,6 +396,54 @@ void bho_potential(int G, const int64_t *offsets, const int32_t *key, int64_t n,
     free(cM);
 }
 
+/* Cloud-to-cloud gravity of the reference's default gravity plugin,
+ * kernels/gravity/cpu_numba.py:8-92 (cpu_numba_kernel), in its loop order:
+ *   same != 0: self_self_numba (:30-47): every pair i < j once; with
+ *              d = x_j - x_i and f = Gc * d * w(|d|^2), row i gets f m_j
+ *              and row j gets -f m_i;
+ *   same == 0: self_other_numba_2 (:49-70) if update_other, else
+ *              self_other_numba_1 (:72-92): for every other row i (outer) and
+ *              self row j (inner), d = x_j - y_i and f = Gc * d * w(|d|^2);
+ *              self row j gets -f M_i and, with update_other, other row i
+ *              gets f m_j.
+ * w = law_w above (the reference's CPU law: r2_cut = 1e-5, eps2 = 0).  Rows
+ * are n x ld doubles with px, py, mass in columns 0, 1, 2.  The sums are
+ * WRITTEN to accA[2 na] and accB[2 nb] (x, y interleaved; accB untouched
+ * unless update_other) for the caller to add, as cpu_numba_kernel adds its
+ * fields to the clouds' accelerations.  Plain FP64 sums. */
+void bho_cloud_gravity(int64_t na, const double *A, int64_t lda, int64_t nb, const double *B,
+                       int64_t ldb, int same, double Gc, double eps2, double r2_cut,
+                       int update_other, double *accA, double *accB)
+{
+    memset(accA, 0, sizeof(double) * 2 * (size_t)na);
+    if (same) {
+        for (int64_t i = 0; i < na; ++i)
+            for (int64_t j = i + 1; j < na; ++j) {
+                double dx = A[j * lda] - A[i * lda], dy = A[j * lda + 1] - A[i * lda + 1];
+                double w = law_w(dx * dx + dy * dy, eps2, r2_cut);
+                double fx = Gc * dx * w, fy = Gc * dy * w;
+                accA[2 * i] += fx * A[j * lda + 2];
+                accA[2 * i + 1] += fy * A[j * lda + 2];
+                accA[2 * j] -= fx * A[i * lda + 2];
+                accA[2 * j + 1] -= fy * A[i * lda + 2];
+            }
+        return;
+    }
+    if (update_other) memset(accB, 0, sizeof(double) * 2 * (size_t)nb);
+    for (int64_t i = 0; i < nb; ++i)
+        for (int64_t j = 0; j < na; ++j) {
+            double dx = A[j * lda] - B[i * ldb], dy = A[j * lda + 1] - B[i * ldb + 1];
+            double w = law_w(dx * dx + dy * dy, eps2, r2_cut);
+            double fx = Gc * dx * w, fy = Gc * dy * w;
+            accA[2 * j] -= fx * B[i * ldb + 2];
+            accA[2 * j + 1] -= fy * B[i * ldb + 2];
+            if (update_other) {
+                accB[2 * i] += fx * A[j * lda + 2];
+                accB[2 * i + 1] += fy * A[j * lda + 2];
+            }
+        }
+}
+
 /* Symplectic-Euler tick, gpu/grid.py:159-174 (g_tick_particles):
  * v += a*dt; a = 0; x += v*dt. Separate multiply and add (no FMA). */
 void bho_tick(int64_t n, double *x, double *y, double *vx, double *vy, double *ax, double *ay,

@@ -20,6 +20,7 @@ References (python-bh/...):
                           barneshut/kernels/gravity/cpu_numba.py:30-92,
                           barneshut/kernels/grid_decomposition/gpu/grid.py:116-268
   direct sum              barneshut/implementations/base.py:16-32
+  cloud-to-cloud gravity  barneshut/kernels/gravity/cpu_numba.py:8-92 (the gravity plugin)
   grid potential          (no reference counterpart: the energy diagnostic of long runs)
   tick                    barneshut/kernels/grid_decomposition/gpu/grid.py:159-174
 """
@@ -71,6 +72,9 @@ def lib():
         L.bho_potential.argtypes = [ctypes.c_int, _i64p, _i32p, ctypes.c_int64, _dp, _dp, _dp,
                                     _dp, ctypes.c_double, _i64p, ctypes.c_int64, ctypes.c_int,
                                     _dp]
+        L.bho_cloud_gravity.argtypes = [ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64, _dp,
+                                        ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -256,6 +260,30 @@ def tick(x, y, vx, vy, ax, ay, dt):
     for a in (x, y, vx, vy, ax, ay):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     lib().bho_tick(len(x), _d(x), _d(y), _d(vx), _d(vy), _d(ax), _d(ay), float(dt))
+
+
+def cloud_gravity(A, B=None, Gc=1.0, eps2=0.0, r2_cut=1e-5, update_other=False):
+    """cpu_numba.py:8-92: the fields cpu_numba_kernel adds to the clouds.
+
+    A, B: (n, ld >= 3) float64 rows (px, py, mass first).  B is None: one
+    cloud (self_self_numba).  Returns (accA, accB) as (n, 2) arrays; accB is
+    None unless B is given and update_other."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    if A.ndim != 2 or A.shape[1] < 3:
+        raise ValueError("rows need shape (n, >= 3)")
+    accA = np.zeros((len(A), 2))
+    same = B is None
+    if same:
+        B = A
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    if B.ndim != 2 or B.shape[1] < 3:
+        raise ValueError("rows need shape (n, >= 3)")
+    upd = bool(update_other) and not same
+    accB = np.zeros((len(B), 2)) if upd else None
+    lib().bho_cloud_gravity(len(A), _d(A), A.shape[1], len(B), _d(B), B.shape[1], int(same),
+                            float(Gc), float(eps2), float(r2_cut), int(upd), _d(accA),
+                            _d(accB) if upd else _d(np.zeros(2)))
+    return accA, accB
 
 
 def interactions(G, counts):

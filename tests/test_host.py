@@ -122,3 +122,46 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_gravity_registry_follows_force_calculation(tmp_path):
+    """gravity/__init__.py:6-41: the choice is read once from [sequential]
+    force_calculation and cached; numba and hip give the HIP kernel, nop a no-op,
+    the reference's other formulations are refused."""
+    from barneshut.kernels import gravity
+    from barneshut.kernels.gravity import hip_cloud
+    ini = tmp_path / "g.ini"
+    try:
+        for fc in ("numba", "hip"):
+            ini.write_text(f"[sequential]\nforce_calculation = {fc}\n")
+            Config.read_file(str(ini))
+            gravity.reset()
+            fn = gravity.get_gravity_kernel()
+            assert fn.__code__ is hip_cloud.get_kernel_function().__code__
+            assert gravity.get_gravity_kernel() is fn              # cached
+        ini.write_text("[sequential]\nforce_calculation = nop\n")
+        Config.read_file(str(ini))
+        gravity.reset()
+        assert gravity.get_gravity_kernel() is gravity.nop
+        assert gravity.nop(1, 2, 3.0, True) is None
+        for fc in ("vect", "blas", "guvectorize-cuda", "pykokkos"):
+            ini.write_text(f"[sequential]\nforce_calculation = {fc}\n")
+            Config.read_file(str(ini))
+            gravity.reset()
+            with pytest.raises(NotImplementedError):
+                gravity.get_gravity_kernel()
+    finally:
+        gravity.reset()
+        Config.read_file(os.path.join(CFG, "default.ini"))
+
+
+def test_cloud_kernel_refuses_bad_rows_before_any_device_call():
+    from barneshut.kernels.gravity import hip_cloud
+    good = np.zeros((4, 10))
+    for bad in (np.zeros((4, 6)), np.zeros((4, 10), np.float32), np.zeros(10)):
+        with pytest.raises(ValueError):
+            hip_cloud._to_device(bad, "self_cloud", None)
+    ro = good.copy()
+    ro.flags.writeable = False
+    with pytest.raises(ValueError):
+        hip_cloud._to_device(ro, "self_cloud", None)

@@ -386,3 +386,72 @@ def test_oracle_refuses_inconsistent_inputs():
         orc.forces(2, off, key, x, x, m, com, targets=np.array([2]))
     ax, ay = orc.forces(2, off, key, x, x, m, com)
     assert np.isfinite(ax).all() and np.isfinite(ay).all()
+
+
+# --------------------------------------------------------------------------
+# cloud-to-cloud gravity (cpu_numba.py:8-92), pinned by closed forms, exact
+# dyadic cutoff cases, the direct sum (itself pinned above) and linearity
+# --------------------------------------------------------------------------
+def _cloud_rows(rng, n, lo=0.0, hi=1.0):
+    r = np.zeros((n, 10))
+    r[:, 0:2] = rng.uniform(lo, hi, (n, 2))
+    r[:, 2] = rng.uniform(1.0, 2.0, n)
+    return r
+
+
+def test_cloud_gravity_two_body_closed_form():
+    A = np.array([[0.0, 0.0, 2.0]])
+    B = np.array([[3.0, 4.0, 5.0]])
+    # a_A = G m_B (x_B - x_A) / r^2 = 1.5 * 5 * (3, 4) / 25; a_B = 1.5 * 2 * (-3, -4) / 25
+    accA, accB = orc.cloud_gravity(A, B, Gc=1.5, update_other=True)
+    assert np.allclose(accA, [[0.9, 1.2]], rtol=1e-15, atol=0)
+    assert np.allclose(accB, [[-0.36, -0.48]], rtol=1e-15, atol=0)
+    accA1, none = orc.cloud_gravity(A, B, Gc=1.5, update_other=False)
+    assert none is None and np.array_equal(accA1, accA)
+    acc, _ = orc.cloud_gravity(np.vstack([A, B]), Gc=1.5)       # one cloud of both rows
+    assert np.allclose(acc, [[0.9, 1.2], [-0.36, -0.48]], rtol=1e-15, atol=0)
+
+
+def test_cloud_gravity_cutoff_on_exact_dyadic_distances():
+    # r^2 = 2^-18 (3.8e-6) <= 1e-5: dropped; r^2 = 2^-16 (1.5e-5): kept, a = m / dx
+    for dx, want in ((2.0 ** -9, 0.0), (2.0 ** -8, 3.0 * 256.0)):
+        A = np.array([[0.25, 0.5, 1.0]])
+        B = np.array([[0.25 + dx, 0.5, 3.0]])
+        acc, _ = orc.cloud_gravity(A, B, Gc=1.0, r2_cut=1e-5)
+        assert acc[0, 0] == want and acc[0, 1] == 0.0
+
+
+def test_cloud_gravity_one_cloud_equals_direct_sum():
+    rng = np.random.default_rng(5)
+    A = _cloud_rows(rng, 300)
+    A[7, :2] = A[3, :2] + 1e-4        # a pair inside the cutoff (r^2 = 2e-8)
+    acc, _ = orc.cloud_gravity(A, Gc=0.7, r2_cut=1e-5)
+    ax, ay = orc.direct(A[:, 0], A[:, 1], A[:, 2], Gc=0.7, r2_cut=1e-5)
+    ref = np.stack([ax, ay], 1)
+    assert np.linalg.norm(acc - ref) <= 1e-13 * np.linalg.norm(ref)
+
+
+def test_cloud_gravity_other_cloud_is_union_minus_self():
+    rng = np.random.default_rng(6)
+    A = _cloud_rows(rng, 200, 0.0, 1.0)
+    B = _cloud_rows(rng, 150, 0.5, 1.5)
+    accA, accB = orc.cloud_gravity(A, B, Gc=2.0, update_other=True)
+    U = np.vstack([A, B])
+    ux, uy = orc.direct(U[:, 0], U[:, 1], U[:, 2], Gc=2.0, r2_cut=1e-5)
+    for acc, rows, sl in ((accA, A, slice(0, 200)), (accB, B, slice(200, 350))):
+        sx, sy = orc.direct(rows[:, 0], rows[:, 1], rows[:, 2], Gc=2.0, r2_cut=1e-5)
+        ref = np.stack([ux[sl] - sx, uy[sl] - sy], 1)
+        assert np.linalg.norm(acc - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_cloud_gravity_conserves_momentum():
+    rng = np.random.default_rng(8)
+    A = _cloud_rows(rng, 120)
+    B = _cloud_rows(rng, 90, 0.2, 0.9)
+    acc, _ = orc.cloud_gravity(A, Gc=1.0)
+    scale = np.abs(A[:, 2:3] * acc).sum()
+    assert np.abs((A[:, 2:3] * acc).sum(0)).max() <= 1e-13 * scale
+    accA, accB = orc.cloud_gravity(A, B, Gc=1.0, update_other=True)
+    tot = (A[:, 2:3] * accA).sum(0) + (B[:, 2:3] * accB).sum(0)
+    scale = np.abs(A[:, 2:3] * accA).sum() + np.abs(B[:, 2:3] * accB).sum()
+    assert np.abs(tot).max() <= 1e-13 * scale

@@ -4,13 +4,17 @@
 
 A cloud is anything with a ``particles`` attribute -- the reference's Cloud
 (internals/cloud.py:39-41), rows in the N x 10 layout of internals/particle.py
--- or a bare array of such rows.  Rows held on the host (numpy) are copied to
-the device (their first 7 columns), evaluated there, and only their ax, ay
-columns are written back; rows held in a float64 device tensor are updated in
-place with no copy.  As in cpu_numba_kernel, the accelerations are added to,
-never reset, and one cloud passed twice means its pairs among themselves.
+-- or a bare array of such rows.  Rows held in a float64 device tensor are
+updated in place with no copy.  Rows held on the host (numpy) go through
+pinned staging memory kept per device: their first 7 columns are packed into
+it, copied to the device in one transfer, evaluated there, and the rows come
+back in one transfer, of which only ax, ay are written into the cloud.  As in
+cpu_numba_kernel the accelerations are added to, never reset, and one cloud
+passed twice means its pairs among themselves.
 """
 from __future__ import annotations
+
+import threading
 
 import numpy as np
 
@@ -19,6 +23,9 @@ from barneshut.kernels import hiplib
 R2_CUT = 1e-5   # cpu_numba.py:36,44,59,67,81,89: pairs with r^2 <= 1e-5 are dropped
 NCOL = 7        # px, py, mass, vx, vy, ax, ay: what the kernel reads or writes
 AX = 5
+
+_stage = {}                 # device index -> (pinned host rows, device rows)
+_stage_lock = threading.Lock()
 
 
 def get_kernel_function(r2_cut=R2_CUT, eps2=0.0):
@@ -32,20 +39,24 @@ def _rows(cloud):
     return cloud.particles if hasattr(cloud, "particles") else cloud
 
 
-def _to_device(rows, name, dev):
-    """(device tensor, host array to write back or None)."""
-    torch = hiplib._torch()
-    if isinstance(rows, torch.Tensor):
-        if not rows.is_cuda:
-            raise ValueError(f"{name}: a tensor of rows must be on the GPU")
-        return rows, None
+def _check_host(rows, name):
     if not isinstance(rows, np.ndarray) or rows.dtype != np.float64 or rows.ndim != 2 \
             or rows.shape[1] < NCOL:
         raise ValueError(f"{name}: rows must be a float64 array of shape (n, >= {NCOL})")
     if not rows.flags.writeable:
         raise ValueError(f"{name}: rows must be writeable (accelerations are added in place)")
-    d = torch.from_numpy(np.ascontiguousarray(rows[:, :NCOL])).to(dev)
-    return d, rows
+
+
+def _staging(dev, nrows):
+    """Pinned host and device buffers of at least nrows x NCOL, grown by doubling."""
+    torch = hiplib._torch()
+    h, d = _stage.get(dev.index, (None, None))
+    if h is None or h.shape[0] < nrows:
+        cap = max(nrows, 2 * (h.shape[0] if h is not None else 0), 4096)
+        h = torch.empty((cap, NCOL), dtype=torch.float64, pin_memory=True)
+        d = torch.empty((cap, NCOL), dtype=torch.float64, device=dev)
+        _stage[dev.index] = (h, d)
+    return h, d
 
 
 def apply_force(self_cloud, other_cloud, G, update_other=False, r2_cut=R2_CUT, eps2=0.0,
@@ -56,16 +67,43 @@ def apply_force(self_cloud, other_cloud, G, update_other=False, r2_cut=R2_CUT, e
     same = self_cloud is other_cloud   # cpu_numba.py:9 compares the clouds themselves
     A = _rows(self_cloud)
     B = A if same else _rows(other_cloud)
-    dev = A.device if isinstance(A, torch.Tensor) else (
-        B.device if isinstance(B, torch.Tensor) else torch.device("cuda",
-                                                                  torch.cuda.current_device()))
-    dA, hA = _to_device(A, "self_cloud", dev)
-    dB, hB = (dA, hA) if same else _to_device(B, "other_cloud", dev)
-    hiplib.cloud_gravity(dA, None if same else dB, G=G, eps2=eps2, r2_cut=r2_cut,
-                         update_other=update_other, stream=stream)
-    if stream is not None and (hA is not None or hB is not None):
-        stream.synchronize()   # the copies below run on the current stream
-    if hA is not None:
-        hA[:, AX:AX + 2] = dA[:, AX:AX + 2].cpu().numpy()
-    if hB is not None and not same and update_other:
-        hB[:, AX:AX + 2] = dB[:, AX:AX + 2].cpu().numpy()
+    tA, tB = isinstance(A, torch.Tensor), isinstance(B, torch.Tensor)
+    for t, name in ((A, "self_cloud"), (B, "other_cloud")):
+        if isinstance(t, torch.Tensor):
+            if not t.is_cuda:
+                raise ValueError(f"{name}: a tensor of rows must be on the GPU")
+        else:
+            _check_host(t, name)
+    if tA and tB:
+        hiplib.cloud_gravity(A, None if same else B, G=G, eps2=eps2, r2_cut=r2_cut,
+                             update_other=update_other, stream=stream)
+        return
+    dev = A.device if tA else (B.device if tB else
+                               torch.device("cuda", torch.cuda.current_device()))
+    s = stream if stream is not None else torch.cuda.current_stream(dev)
+    host = [] if tA else [A]
+    if not same and not tB:
+        host.append(B)
+    with _stage_lock:
+        nrows = sum(len(r) for r in host)
+        h, d = _staging(dev, nrows)
+        hv = h.numpy()
+        views, o = [], 0
+        for r in host:
+            hv[o:o + len(r)] = r[:, :NCOL]
+            views.append(d[o:o + len(r)])
+            o += len(r)
+        with torch.cuda.stream(s):
+            d[:nrows].copy_(h[:nrows], non_blocking=True)
+        dA = A if tA else views[0]
+        dB = dA if same else (B if tB else views[-1])
+        hiplib.cloud_gravity(dA, None if same else dB, G=G, eps2=eps2, r2_cut=r2_cut,
+                             update_other=update_other, stream=s)
+        with torch.cuda.stream(s):
+            h[:nrows].copy_(d[:nrows], non_blocking=True)
+        s.synchronize()
+        o = 0
+        for r in host:
+            if r is A or update_other:
+                r[:, AX:AX + 2] = hv[o:o + len(r), AX:AX + 2]
+            o += len(r)

@@ -12,7 +12,7 @@
 // with r^2 <= r2_cut, as the reference's CPU kernel does at 1e-5).
 //
 // One 256-thread block owns P * G' consecutive targets (G' = 256 / L groups of
-// L lanes, P targets per lane).  Sources are staged through LDS in tiles of
+// L lanes, P = 1 or 2 targets per lane).  Sources are staged through LDS in tiles of
 // 256 rows; lane l of a group takes the tile rows l, l + L, l + 2L, ... for
 // its P targets, and the L partial sums of a target are added by a fixed
 // tree.  The shape (P, L) is a function of the two cloud sizes only, so a
@@ -49,18 +49,30 @@ struct CloudArgs {
     double G, eps2, r2cut;
 };
 
-template <int LAW, int P, int MASK>
+// m w(r^2) of the law: LAW 1 drops r^2 <= r2cut; SOFT 0 has eps2 = 0 (the
+// reference laws), so r^2 + eps2 is not formed.
+template <int LAW, int SOFT>
+__device__ __forceinline__ double cloud_w(double wx, double wy, double m, double eps2,
+                                          double r2cut)
+{
+    const double s2 = fma(wx, wx, wy * wy);
+    const double w = wdiv64(m, SOFT ? s2 + eps2 : s2);
+    return LAW ? ((s2 > r2cut) ? w : 0.0) : w;
+}
+
+template <int LAW, int SOFT, int P, int MASK>
 __device__ __forceinline__ void cloud_tile(const double* sx, const double* sy, const double* sm,
                                            int cnt, int lane, int L, int64_t t0,
                                            const int64_t* ti, const double* xi, const double* yi,
                                            double* ax, double* ay, double eps2, double r2cut)
 {
+#pragma unroll 2
     for (int k = lane; k < cnt; k += L) {
         const double x = sx[k], y = sy[k], m = sm[k];
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const double wx = x - xi[p], wy = y - yi[p];
-            double w = near_w64<LAW, 1>(wx, wy, m, eps2, r2cut);
+            double w = cloud_w<LAW, SOFT>(wx, wy, m, eps2, r2cut);
             if (MASK) w = (t0 + k == ti[p]) ? 0.0 : w;
             ax[p] = fma(wx, w, ax[p]);
             ay[p] = fma(wy, w, ay[p]);
@@ -68,7 +80,7 @@ __device__ __forceinline__ void cloud_tile(const double* sx, const double* sy, c
     }
 }
 
-template <int LAW, int P>
+template <int LAW, int SOFT, int P>
 __global__ __launch_bounds__(CL_THREADS) void k_cloud(CloudArgs A)
 {
     // tiles use 3 x 256 doubles, the final tree 2 P x 256
@@ -107,11 +119,11 @@ __global__ __launch_bounds__(CL_THREADS) void k_cloud(CloudArgs A)
         __syncthreads();
         const int cnt = (int)std::min<int64_t>(CL_THREADS, A.ns - t0);
         if (A.mask_self)
-            cloud_tile<LAW, P, 1>(sx, sy, sm, cnt, lane, L, t0, ti, xi, yi, ax, ay, A.eps2,
-                                  A.r2cut);
+            cloud_tile<LAW, SOFT, P, 1>(sx, sy, sm, cnt, lane, L, t0, ti, xi, yi, ax, ay,
+                                        A.eps2, A.r2cut);
         else
-            cloud_tile<LAW, P, 0>(sx, sy, sm, cnt, lane, L, t0, ti, xi, yi, ax, ay, A.eps2,
-                                  A.r2cut);
+            cloud_tile<LAW, SOFT, P, 0>(sx, sy, sm, cnt, lane, L, t0, ti, xi, yi, ax, ay,
+                                        A.eps2, A.r2cut);
     }
 
     __syncthreads();
@@ -143,13 +155,13 @@ __global__ __launch_bounds__(CL_THREADS) void k_cloud(CloudArgs A)
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// Shape of one launch from the two sizes: 2 targets per lane once there are
-// enough targets to fill CL_MIN_BLOCKS blocks with them, and more lanes per
-// target while blocks are fewer than CL_MIN_BLOCKS and every lane keeps at
-// least CL_MIN_PER_LANE source rows.
+// Shape of one launch from the two sizes: 2 targets per lane (one LDS read
+// and the loop's bookkeeping serve both) once there are 2 per thread of a
+// block, and more lanes per target while blocks are fewer than CL_MIN_BLOCKS
+// and every lane keeps at least CL_MIN_PER_LANE source rows.
 void cloud_shape(int64_t nt, int64_t ns, int* P, int* llog2)
 {
-    *P = nt >= 2LL * CL_THREADS * CL_MIN_BLOCKS ? 2 : 1;
+    *P = nt >= 2LL * CL_THREADS ? 2 : 1;
     int l = 0;
     while (l < 8 && cdiv(nt, (int64_t)*P * (CL_THREADS >> l)) < CL_MIN_BLOCKS &&
            ((int64_t)CL_MIN_PER_LANE << (l + 1)) <= ns)
@@ -176,14 +188,13 @@ int launch_cloud(const bh_cloud* tgt, const bh_cloud* src, int mask_self, const 
     A.llog2 = l;
     const int64_t blocks = cdiv(A.nt, (int64_t)P * (CL_THREADS >> l));
     if (blocks > 0x7fffffffLL) return bh_ctx_fail(BH_EINVAL, "cloud too large for one launch");
-    const bool cut = law->r2_cut > 0.0;
-    if (P == 2) {
-        if (cut) hipLaunchKernelGGL((k_cloud<1, 2>), dim3((unsigned)blocks), dim3(CL_THREADS), 0, st, A);
-        else     hipLaunchKernelGGL((k_cloud<0, 2>), dim3((unsigned)blocks), dim3(CL_THREADS), 0, st, A);
-    } else {
-        if (cut) hipLaunchKernelGGL((k_cloud<1, 1>), dim3((unsigned)blocks), dim3(CL_THREADS), 0, st, A);
-        else     hipLaunchKernelGGL((k_cloud<0, 1>), dim3((unsigned)blocks), dim3(CL_THREADS), 0, st, A);
-    }
+    typedef void (*Kern)(CloudArgs);
+    // [cutoff][softened][P == 2]
+    static const Kern table[2][2][2] = {
+        {{k_cloud<0, 0, 1>, k_cloud<0, 0, 2>}, {k_cloud<0, 1, 1>, k_cloud<0, 1, 2>}},
+        {{k_cloud<1, 0, 1>, k_cloud<1, 0, 2>}, {k_cloud<1, 1, 1>, k_cloud<1, 1, 2>}}};
+    const Kern k = table[law->r2_cut > 0.0][law->eps2 > 0.0][P == 2];
+    hipLaunchKernelGGL(k, dim3((unsigned)blocks), dim3(CL_THREADS), 0, st, A);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : bh_ctx_fail(BH_EHIP, hipGetErrorString(e));
 }

@@ -37,7 +37,9 @@ learned), 512 independent systems stepped by one graph launch per step, the
 reference's CPU-runnable case (16384 uniform particles, 32^2, eps = 1e-3: the
 oracle's full step on the host cores and the GPU step), and the reference's
 own logged configuration (N = 10M uniform in [0,499)^2, 120 x 120 grid, FP64,
-G = 1, dt = 0.005) against its published 166.837 s/step.
+G = 1, dt = 0.005) against its published 166.837 s/step, and the gravity
+plugin's cloud-to-cloud kernel (one cloud of 131,072 particles; a 64-particle
+leaf against 16,384 COMs, on device and on host rows).
 
 Prints one JSON line.
 """
@@ -415,6 +417,78 @@ def reference_config_line(steps, tuning64):
     return out
 
 
+def gravity_plugin_line():
+    """The gravity plugin (bh_cloud_gravity, cpu_numba.py:8-92), FP64, the
+    reference CPU law (r^2 <= 1e-5 dropped):
+    * one cloud of 131,072 uniform particles (device rows): every ordered pair,
+      1.718e10 terms per call, timed with HIP events on the launch stream;
+    * a leaf of 64 particles against a COM pseudo-cloud of 16,384 rows -- the
+      reference's per-leaf far-field call (seq/evaluation.py:95-98) -- on
+      device rows, and on numpy rows (the host copies included).
+    The CPU side is gravity_plugin_cpu (timed with the other baselines)."""
+    import torch
+    from barneshut.kernels import hiplib
+    from barneshut.kernels.gravity import hip_cloud
+    rng = np.random.default_rng(21)
+
+    def rows(n, lo=0.0, hi=1.0):
+        return _cloud_rows(rng, n, lo, hi)
+
+    def timed(fn, calls):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(calls):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1e-3 / calls
+
+    n = 131072
+    big = torch.from_numpy(rows(n)).to("cuda")
+    t_big = timed(lambda: hiplib.cloud_gravity(big), 5)
+    terms = float(n) * (n - 1)
+    leaf, coms = rows(64, 0.0, 0.01), rows(16384)
+    dl, dc = torch.from_numpy(leaf).to("cuda"), torch.from_numpy(coms).to("cuda")
+    t_leaf = timed(lambda: hiplib.cloud_gravity(dl, dc), 200)
+    t0 = time.perf_counter()
+    for _ in range(50):
+        hip_cloud.apply_force(leaf, coms, 1.0, False)
+    t_host = (time.perf_counter() - t0) / 50
+    ach = FLOP_PER_INTERACTION * terms / t_big / 1e12
+    return {"kernel": "k_cloud (FP64, refined reciprocal)",
+            "one_cloud": {"n": n, "terms": terms, "ms_per_call": t_big * 1e3,
+                          "terms_per_s": terms / t_big, "achieved_tflops": ach,
+                          "frac_of_fp64_peak": ach / FP64_PEAK_TFLOPS},
+            "leaf_vs_coms": {"leaf": 64, "coms": 16384, "device_us_per_call": t_leaf * 1e6,
+                             "host_rows_us_per_call": t_host * 1e6}}
+
+
+def _cloud_rows(rng, n, lo=0.0, hi=1.0):
+    r = np.zeros((n, 10))
+    r[:, 0:2] = rng.uniform(lo, hi, (n, 2))
+    r[:, 2] = rng.uniform(1.0, 2.0, n)
+    return r
+
+
+def gravity_plugin_cpu(n_gpu_cloud):
+    """The oracle's restatement of cpu_numba's one-cloud loop (each pair once,
+    both rows updated) on one host core, on a cloud of 8,192 uniform rows;
+    extrapolated per pair to the GPU line's cloud."""
+    from oracle import oracle as orc
+    ns = 8192
+    sample = _cloud_rows(np.random.default_rng(22), ns)
+    t0 = time.perf_counter()
+    orc.cloud_gravity(sample)
+    t = time.perf_counter() - t0
+    per_pair = t / (ns * (ns - 1) / 2.0)
+    return {"kind": "port (oracle/bh_oracle.c, the loops of cpu_numba.py)", "cores": 1,
+            "sample": f"one cloud of {ns} rows, {ns * (ns - 1) // 2} pairs, {t:.2f} s",
+            "ns_per_pair": per_pair * 1e9,
+            "one_cloud_s_extrapolated": per_pair * n_gpu_cloud * (n_gpu_cloud - 1) / 2.0}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
@@ -587,8 +661,12 @@ def main():
         out["reference_config"] = reference_config_line(3, tuning64)
         log("reference CPU-runnable case")
         out["c1"] = c1_line(5)
+        log("gravity plugin")
+        out["gravity_plugin"] = gravity_plugin_line()
     if not args.skip_cpu:
         log("cpu baseline")
+        if "gravity_plugin" in out:
+            out["gravity_plugin"]["cpu"] = gravity_plugin_cpu(out["gravity_plugin"]["one_cloud"]["n"])
         cb = cpu_baseline(d, G)
         out["placement_bit_exact"] = cb.pop("placement_check")
         out["cpu_baseline"] = cb

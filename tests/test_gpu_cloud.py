@@ -244,3 +244,26 @@ def test_refusals():
         hiplib.cloud_gravity(good, G=float("nan"))
     with pytest.raises(hiplib.BhError):
         hiplib.cloud_gravity(good, r2_cut=-1.0)
+
+
+def test_host_rows_mixed_with_device_rows_and_growing_staging():
+    """Host rows against device rows (either side), and host calls whose sizes
+    grow past the staging buffer, each equal to the all-device call."""
+    import torch
+    from barneshut.kernels.gravity import hip_cloud
+    rng = np.random.default_rng(15)
+    for na, nb in ((30, 40), (5000, 3000), (9000, 200)):
+        A, B = _rows(rng, na), _rows(rng, nb, 0.2, 1.2)
+        ref = _call(A, B, G=1.5, update_other=True)
+        hA, dB = A.copy(), _dev(B)
+        hip_cloud.apply_force(hA, dB, 1.5, True)
+        torch.cuda.synchronize()
+        assert np.array_equal(hA, ref[0]) and np.array_equal(dB.cpu().numpy(), ref[1])
+        dA, hB = _dev(A), B.copy()
+        hip_cloud.apply_force(dA, hB, 1.5, True)
+        torch.cuda.synchronize()
+        assert np.array_equal(dA.cpu().numpy(), ref[0]) and np.array_equal(hB, ref[1])
+        hA, hB = A.copy(), B.copy()
+        hip_cloud.apply_force(hA, hB, 1.5, False)
+        one = _call(A, B, G=1.5, update_other=False)
+        assert np.array_equal(hA, one[0]) and np.array_equal(hB, B)

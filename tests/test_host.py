@@ -158,10 +158,10 @@ def test_gravity_registry_follows_force_calculation(tmp_path):
 def test_cloud_kernel_refuses_bad_rows_before_any_device_call():
     from barneshut.kernels.gravity import hip_cloud
     good = np.zeros((4, 10))
-    for bad in (np.zeros((4, 6)), np.zeros((4, 10), np.float32), np.zeros(10)):
-        with pytest.raises(ValueError):
-            hip_cloud._to_device(bad, "self_cloud", None)
     ro = good.copy()
     ro.flags.writeable = False
-    with pytest.raises(ValueError):
-        hip_cloud._to_device(ro, "self_cloud", None)
+    for bad in (np.zeros((4, 6)), np.zeros((4, 10), np.float32), np.zeros(10), ro):
+        with pytest.raises(ValueError):
+            hip_cloud.apply_force(bad, good, 1.0)
+        with pytest.raises(ValueError):
+            hip_cloud.apply_force(good, bad, 1.0, True)

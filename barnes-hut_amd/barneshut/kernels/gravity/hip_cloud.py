@@ -14,11 +14,22 @@ passed twice means its pairs among themselves.
 """
 from __future__ import annotations
 
+import os
 import threading
 
 import numpy as np
 
-from barneshut.kernels import hiplib
+try:
+    from .. import hiplib
+except ImportError:
+    # loaded by file path from another project's registry (whose own package
+    # may also be called barneshut): take hiplib from this file's tree
+    import importlib.util as _ilu
+    _spec = _ilu.spec_from_file_location(
+        "bh_hip_hiplib", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                      "hiplib.py"))
+    hiplib = _ilu.module_from_spec(_spec)
+    _spec.loader.exec_module(hiplib)
 
 R2_CUT = 1e-5   # cpu_numba.py:36,44,59,67,81,89: pairs with r^2 <= 1e-5 are dropped
 NCOL = 7        # px, py, mass, vx, vy, ax, ay: what the kernel reads or writes

@@ -165,3 +165,28 @@ def test_cloud_kernel_refuses_bad_rows_before_any_device_call():
             hip_cloud.apply_force(bad, good, 1.0)
         with pytest.raises(ValueError):
             hip_cloud.apply_force(good, bad, 1.0, True)
+
+
+def test_plugin_loads_by_path_beside_another_barneshut_package(tmp_path):
+    """The registry branch INTEGRATION.md shows for the reference: hip_cloud.py
+    loaded by file path in a process whose `barneshut` package is another
+    project's (a stand-in here) binds this tree's hiplib, not that package."""
+    pkg = tmp_path / "barneshut" / "kernels" / "gravity"
+    pkg.mkdir(parents=True)
+    for d in (tmp_path / "barneshut", tmp_path / "barneshut" / "kernels", pkg):
+        (d / "__init__.py").write_text("")
+    path = os.path.join(ROOT, "barnes-hut_amd", "barneshut", "kernels", "gravity", "hip_cloud.py")
+    code = ("import importlib.util, sys\n"
+            "import barneshut\n"
+            "spec = importlib.util.spec_from_file_location('bh_hip_cloud', sys.argv[1])\n"
+            "mod = importlib.util.module_from_spec(spec)\n"
+            "spec.loader.exec_module(mod)\n"
+            "fn = mod.get_kernel_function()\n"
+            "print(barneshut.__file__)\n"
+            "print(mod.hiplib.LIB_PATH)\n")
+    out = subprocess.run([sys.executable, "-c", code, path], capture_output=True, text=True,
+                         cwd=str(tmp_path), env={**os.environ, "PYTHONPATH": str(tmp_path)})
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.split()
+    assert lines[0].startswith(str(tmp_path))
+    assert lines[1] == os.path.join(ROOT, "barnes-hut_amd", "csrc", "libbh_hip.so")

@@ -6,9 +6,11 @@ A cloud is anything with a ``particles`` attribute -- the reference's Cloud
 (internals/cloud.py:39-41), rows in the N x 10 layout of internals/particle.py
 -- or a bare array of such rows.  Rows held in a float64 device tensor are
 updated in place with no copy.  Rows held on the host (numpy) go through
-pinned staging memory kept per device: their first 7 columns are packed into
-it, copied to the device in one transfer, evaluated there, and the rows come
-back in one transfer, of which only ax, ay are written into the cloud.  As in
+pinned staging memory kept per device: their positions and masses are packed
+into it (accelerations zeroed), copied to the device in one transfer,
+evaluated there, and the sums come back in one transfer and are added to the
+cloud's ax, ay -- one add per cloud in turn, so clouds that are overlapping
+views of one array both receive theirs, as in cpu_numba_kernel.  As in
 cpu_numba_kernel the accelerations are added to, never reset, and one cloud
 passed twice means its pairs among themselves.
 """
@@ -101,7 +103,8 @@ def apply_force(self_cloud, other_cloud, G, update_other=False, r2_cut=R2_CUT, e
         hv = h.numpy()
         views, o = [], 0
         for r in host:
-            hv[o:o + len(r)] = r[:, :NCOL]
+            hv[o:o + len(r), :AX] = r[:, :AX]
+            hv[o:o + len(r), AX:AX + 2] = 0.0   # the device forms the sums alone
             views.append(d[o:o + len(r)])
             o += len(r)
         with torch.cuda.stream(s):
@@ -110,11 +113,15 @@ def apply_force(self_cloud, other_cloud, G, update_other=False, r2_cut=R2_CUT, e
         dB = dA if same else (B if tB else views[-1])
         hiplib.cloud_gravity(dA, None if same else dB, G=G, eps2=eps2, r2_cut=r2_cut,
                              update_other=update_other, stream=s)
-        with torch.cuda.stream(s):
-            h[:nrows].copy_(d[:nrows], non_blocking=True)
+        # staged rows that receive sums: all of them with update_other, else only
+        # self's (staged first) if self is on the host
+        back = nrows if update_other else (0 if tA else len(A))
+        if back:
+            with torch.cuda.stream(s):
+                h[:back].copy_(d[:back], non_blocking=True)
         s.synchronize()
         o = 0
-        for r in host:
+        for r in host:   # `accelerations += acc` per cloud, as cpu_numba.py:13,24-25,28
             if r is A or update_other:
-                r[:, AX:AX + 2] = hv[o:o + len(r), AX:AX + 2]
+                r[:, AX:AX + 2] += hv[o:o + len(r), AX:AX + 2]
             o += len(r)

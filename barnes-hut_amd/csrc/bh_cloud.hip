@@ -14,8 +14,9 @@
 // One 256-thread block owns P * G' consecutive targets (G' = 256 / L groups of
 // L lanes, P = 1 or 2 targets per lane).  Sources are staged through LDS in tiles of
 // 256 rows; lane l of a group takes the tile rows l, l + L, l + 2L, ... for
-// its P targets, and the L partial sums of a target are added by a fixed
-// tree.  The shape (P, L) is a function of the two cloud sizes only, so a
+// its P targets (with one group per block, L = 256, the lanes read their rows
+// straight from memory instead), and the L partial sums of a target are added
+// by a fixed tree.  The shape (P, L) is a function of the two cloud sizes only, so a
 // result never depends on timing.  FP64 throughout; each weight m / q by the
 // refined hardware reciprocal (wdiv64, within about one ulp of the quotient).
 
@@ -80,6 +81,27 @@ __device__ __forceinline__ void cloud_tile(const double* sx, const double* sy, c
     }
 }
 
+// Source rows lane, lane + 256, ... read straight from memory (L = 256).
+template <int LAW, int SOFT, int P, int MASK>
+__device__ __forceinline__ void cloud_direct(const CloudArgs& A, int lane, const int64_t* ti,
+                                             const double* xi, const double* yi, double* ax,
+                                             double* ay)
+{
+#pragma unroll 4
+    for (int64_t j = lane; j < A.ns; j += CL_THREADS) {
+        const double* r = A.src + j * A.lds;
+        const double x = r[COL_X], y = r[COL_Y], m = r[COL_M];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double wx = x - xi[p], wy = y - yi[p];
+            double w = cloud_w<LAW, SOFT>(wx, wy, m, A.eps2, A.r2cut);
+            if (MASK) w = (j == ti[p]) ? 0.0 : w;
+            ax[p] = fma(wx, w, ax[p]);
+            ay[p] = fma(wy, w, ay[p]);
+        }
+    }
+}
+
 template <int LAW, int SOFT, int P>
 __global__ __launch_bounds__(CL_THREADS) void k_cloud(CloudArgs A)
 {
@@ -107,7 +129,15 @@ __global__ __launch_bounds__(CL_THREADS) void k_cloud(CloudArgs A)
         ay[p] = 0.0;
     }
 
-    for (int64_t t0 = 0; t0 < A.ns; t0 += CL_THREADS) {
+    if (groups == 1) {
+        // one target group per block: a tile would serve only its P targets,
+        // so each lane reads its own source rows (unrolled, loads in flight)
+        if (A.mask_self)
+            cloud_direct<LAW, SOFT, P, 1>(A, lane, ti, xi, yi, ax, ay);
+        else
+            cloud_direct<LAW, SOFT, P, 0>(A, lane, ti, xi, yi, ax, ay);
+    }
+    for (int64_t t0 = 0; groups > 1 && t0 < A.ns; t0 += CL_THREADS) {
         __syncthreads();   // every lane is done with the previous tile
         const int64_t j = t0 + tid;
         if (j < A.ns) {

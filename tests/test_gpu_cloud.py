@@ -267,3 +267,24 @@ def test_host_rows_mixed_with_device_rows_and_growing_staging():
         hip_cloud.apply_force(hA, hB, 1.5, False)
         one = _call(A, B, G=1.5, update_other=False)
         assert np.array_equal(hA, one[0]) and np.array_equal(hB, B)
+
+
+def test_host_clouds_that_overlap_receive_both_sums():
+    """Two clouds that are overlapping views of one array, both sides updated:
+    each cloud's field is added in turn (cpu_numba.py:24-25), so the shared
+    rows receive both sums."""
+    from barneshut.kernels.gravity import hip_cloud
+    rng = np.random.default_rng(16)
+    W = _rows(rng, 600)
+    start = W.copy()
+    A, B = W[:400], W[300:]
+    hip_cloud.apply_force(A, B, 1.0, True)
+    accA, accB = orc.cloud_gravity(start[:400], start[300:], Gc=1.0, update_other=True)
+    want = start[:, 5:7].copy()
+    want[:400] += accA
+    want[300:] += accB
+    mag = np.zeros(600)
+    mag[:400] += _magnitudes(start[:400], start[300:], 1.0, R2_CUT, 0.0, False)
+    mag[300:] += _magnitudes(start[300:], start[:400], 1.0, R2_CUT, 0.0, False)
+    err = np.abs(W[:, 5:7] - want).max(1)
+    assert np.all(err <= 1e-12 * mag)

@@ -383,6 +383,32 @@ def batched_line(steps, model_path, seed=0):
     return out
 
 
+# fixed launch shapes of the batched line's shape sweep (--batched-shapes)
+BATCHED_SWEEP = (dict(p=1, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0),
+                 dict(p=2, yspan=512, waves_per_cu=32, tail_split=1, tail_items=0),
+                 dict(p=2, yspan=512, waves_per_cu=32, tail_split=16, tail_items=1 << 20),
+                 dict(p=1, yspan=512, waves_per_cu=16, tail_split=1, tail_items=0))
+
+
+def batched_shapes(steps, seed=0):
+    """The batched line's workload (static order, graph replay) under each
+    fixed launch shape of BATCHED_SWEEP: ms per step, for comparing shapes
+    (e.g. under rocprofv3 counters)."""
+    from barneshut import inputs
+    from barneshut.batched import BatchedSystems
+    sizes, grids = inputs.batched_sizes(512, 1024, 262144, seed=seed)
+    data = inputs.batched_uniform(sizes, seed=seed)
+    out = []
+    for tun in BATCHED_SWEEP:
+        b = BatchedSystems(sizes, grids, data, (GC, 1e-12, 0.0), "fp32", DT, graph=True,
+                           tuning=tun)
+        sec = b.run(steps, warmup=1)
+        b.close()
+        log(f"batched {tun}: {sec * 1e3:.2f} ms/step")
+        out.append({"tuning": tun, "ms_per_step": sec * 1e3})
+    return out
+
+
 def reference_config_line(steps, tuning64):
     """The reference's logged run (python-bh/data/frontera/nbody_out_singlegpu.dat):
     N = 10,000,000 uniform in [0,499)^2 (bin/gen_input.py), 120 x 120 grid,
@@ -517,6 +543,8 @@ def main():
     ap.add_argument("--batched-model", default=os.path.join(PKG, "costmodels", "batched512.npz"))
     ap.add_argument("--c4-steps", type=int, default=5,
                     help="timed steps of each cost-model-workload path (0 = skip the block)")
+    ap.add_argument("--batched-shapes", action="store_true",
+                    help="only time the batched line under fixed launch shapes (one JSON line)")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-static", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
@@ -539,6 +567,10 @@ def main():
 
     import torch
     from barneshut import inputs
+
+    if args.batched_shapes:
+        print(json.dumps({"batched_shapes": batched_shapes(args.steps)}), flush=True)
+        return
 
     if args.calibrate_copy_mib:
         src = torch.ones(args.calibrate_copy_mib * (1 << 18), dtype=torch.float32, device="cuda")

@@ -332,20 +332,24 @@ def c1_line(steps):
     g = eng.fetch()
     e_gpu = float(np.sqrt(((g["ax"] - ex) ** 2 + (g["ay"] - ey) ** 2).sum()
                           / (ex ** 2 + ey ** 2).sum()))
-    eng.load(d)
-    eng.step(gc, eps2, 0.0, "fp64", dt)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        eng.step(gc, eps2, 0.0, "fp64", dt)
-    torch.cuda.synchronize()
-    t_gpu = (time.perf_counter() - t0) / steps
-    eng.check()
+    t_gpu = {}
+    for graph in (False, True):   # kernel by kernel, and one graph replay per step
+        eng.load(d)
+        eng.step(gc, eps2, 0.0, "fp64", dt, graph=graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.step(gc, eps2, 0.0, "fp64", dt, graph=graph)
+        torch.cuda.synchronize()
+        t_gpu[graph] = (time.perf_counter() - t0) / steps
+        eng.check()
     eng.close()
     return {"n": n, "grid": G, "G": gc, "eps2": eps2, "dt": dt,
             "cpu_full_step_s": t_cpu, "cpu_value": n / t_cpu / 1e6, "cpu_cores": cores,
             "cpu_full_step_1thread_s": t_cpu1, "cpu_kind": "port (oracle/bh_oracle.c)",
-            "gpu_fp64_ms_per_step": t_gpu * 1e3, "gpu_value": n / t_gpu / 1e6,
+            "gpu_fp64_ms_per_step": t_gpu[False] * 1e3, "gpu_value": n / t_gpu[False] / 1e6,
+            "gpu_fp64_graph_ms_per_step": t_gpu[True] * 1e3,
+            "gpu_graph_value": n / t_gpu[True] / 1e6,
             "method_error_vs_direct_sum": {"oracle": e_orc, "gpu_fp64": e_gpu}}
 
 
@@ -692,7 +696,7 @@ def main():
         log("reference configuration")
         out["reference_config"] = reference_config_line(3, tuning64)
         log("reference CPU-runnable case")
-        out["c1"] = c1_line(5)
+        out["c1"] = c1_line(50)
         log("gravity plugin")
         out["gravity_plugin"] = gravity_plugin_line()
     if not args.skip_cpu:

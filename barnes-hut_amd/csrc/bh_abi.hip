@@ -866,11 +866,9 @@ static int step_graph_impl(void* ctx, const bh_particles* a, const bh_particles*
         c->gkey[slot] = k;
     }
     hipStream_t st = (hipStream_t)stream;
-    HIP_TRY(hipEventRecord(c->ev_in, st));
-    HIP_TRY(hipStreamWaitEvent(c->cap_stream, c->ev_in, 0));
-    HIP_TRY(hipGraphLaunch(c->gexec[slot], c->cap_stream));
-    HIP_TRY(hipEventRecord(c->ev_out, c->cap_stream));
-    HIP_TRY(hipStreamWaitEvent(st, c->ev_out, 0));
+    // the graph was captured on the context's own stream and replays on the
+    // caller's: ordered after the caller's earlier work, before its later work
+    HIP_TRY(hipGraphLaunch(c->gexec[slot], st));
     return BH_OK;
 }
 

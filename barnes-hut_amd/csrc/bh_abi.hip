@@ -100,8 +100,7 @@ struct bh_ctx_s {
     double* rc_blk = nullptr;      // per block: sum of M^2
     double* rc_bfar = nullptr;     // per block: its far-block part of the bound
     int64_t nblocks = 0;
-    hipStream_t cap_stream = nullptr;
-    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipStream_t cap_stream = nullptr;   // graph capture only; replays run on the caller's stream
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     GraphKey gkey[2];
     int gnext = 0;
@@ -139,6 +138,18 @@ static int scratch_address(int dev, unsigned char** out, int* num_cu);
 static inline unsigned grid_for(int64_t n, int bs, int64_t cap = 1 << 20)
 {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(cdiv(n, bs), cap));
+}
+
+// Workgroups of a persistent force-type kernel: waves_per_cu per CU, but no
+// more than the queue is expected to hold -- about one item per 64 P
+// particles plus two per grid column, each split into `split` slices -- so a
+// small system does not pay for launching thousands of waves that find the
+// queue empty.  Any size is correct: every wave takes items until none is left.
+static unsigned persistent_grid(const Ctx* c, int split)
+{
+    const int64_t want = (int64_t)c->num_cu * c->tune.waves_per_cu;
+    const int64_t queue = (cdiv(c->N, 64LL * c->tune.p) + 2 * c->ncols) * split;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(want, queue));
 }
 
 // Checks the system sizes and lays the systems out in the global particle,
@@ -292,11 +303,9 @@ int bh_create(int device, void** ctx_out)
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cu > 0)
         c->num_cu = cu;
-    if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
-        return fail(BH_EHIP, "stream/event creation failed");
+        return fail(BH_EHIP, "stream creation failed");
     }
     // the allocation-free step's static scratch: its device address on this GPU
     unsigned char* sp = nullptr;
@@ -319,8 +328,6 @@ int bh_destroy(void* ctx)
     free_all(c);
     if (c->part) (void)hipFree(c->part);
     if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
-    if (c->ev_in) (void)hipEventDestroy(c->ev_in);
-    if (c->ev_out) (void)hipEventDestroy(c->ev_out);
     delete c;
     return BH_OK;
 }
@@ -677,8 +684,7 @@ int bh_force_pass_part(void* ctx, const bh_particles* p, const bh_law* law, int3
     if (part == 0) c->rc_ready = 0;   // until the last part: no recompute of a partial pass
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->stats + ST_QUEUE, (int64_t)1, 0);
     void* args[] = {&fa};
-    unsigned grid = (unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu);
-    HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(64), args, 0, st));
+    HIP_TRY(hipLaunchKernel(fn, dim3(persistent_grid(c, fa.split)), dim3(64), args, 0, st));
     if (part < nparts - 1) {   // the slice combine and the recompute follow the last part
         LAUNCH_CHECK("evaluate");
         return BH_OK;
@@ -1065,7 +1071,7 @@ int bh_energy_grid(void* ctx, const bh_particles* p, double G, double eps2, doub
     pa.eps2 = eps2;
     pa.u = u;
     pa.k = u + c->N;
-    const dim3 grid((unsigned)((int64_t)c->num_cu * c->tune.waves_per_cu));
+    const dim3 grid(persistent_grid(c, 1));
     if (c->tune.p == 1) hipLaunchKernelGGL(k_potential<1>, grid, dim3(64), 0, st, pa);
     else if (c->tune.p == 2) hipLaunchKernelGGL(k_potential<2>, grid, dim3(64), 0, st, pa);
     else hipLaunchKernelGGL(k_potential<4>, grid, dim3(64), 0, st, pa);

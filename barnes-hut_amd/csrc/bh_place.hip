@@ -525,20 +525,41 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
         lend = max(ql + 1, ub);
     }
     __syncthreads();
-    const int64_t wend = min(lend, w0 + 2 * SORT_W);
-    for (int t = threadIdx.x; t < 2 * SORT_W; t += 256) {
-        int64_t q = w0 + t;
-        if (q < wend) {
-            lkx[t] = pxkey[q];
-            lky[t] = pykey[q];
-        }
+    const int64_t wend = min(lend, w0 + 2 * SORT_W);   // > w0: the window holds a row
+    // Each thread's rows t = threadIdx.x + 256 r.  Every load below reads a
+    // row clamped into [w0, wend), so it is always in bounds and needs no
+    // branch: the loads of all rows are issued before the first is waited on
+    // (a guarded load per row would wait for each one in turn).
+    constexpr int RPT = 2 * SORT_W / 256;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int t = threadIdx.x + 256 * r;
+        const int64_t q = min(w0 + t, wend - 1);
+        // slots past wend get the last row's keys; no rank reads them (every
+        // unit this block ranks ends at or before wend)
+        lkx[t] = pxkey[q];
+        lky[t] = pykey[q];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < 2 * SORT_W; t += 256) {
-        int64_t q = w0 + t;
+    // the rows' cells and source rows, then the cells' bounds
+    int kr[RPT], er[RPT];
+    int64_t ar[RPT], br[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int64_t q = min(w0 + threadIdx.x + 256 * r, wend - 1);
+        kr[r] = key_sorted[q];
+        er[r] = idx_in[q];
+    }
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        ar[r] = offsets[kr[r]];
+        br[r] = offsets[kr[r] + 1];
+    }
+    for (int r = 0; r < RPT; ++r) {
+        const int t = threadIdx.x + 256 * r;
+        const int64_t q = w0 + t;
         if (q >= wend) break;
-        int k = key_sorted[q];
-        int64_t a = offsets[k], b = offsets[k + 1];
+        const int64_t a = ar[r], b = br[r];
         int64_t ua, ub;
         if (b - a <= SORT_W) {
             ua = a;
@@ -549,7 +570,7 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
             ub = min(b, ua + SORT_W);
         }
         if (ua < w0 || ua >= w0 + SORT_W) continue;   // unit owned by another block
-        const int e = idx_in[q];
+        const int e = er[r];
         const uint64_t x = lkx[t], xy = lky[t];
         int rank = 0;
         for (int j = (int)(ua - w0); j < (int)(ub - w0); ++j) {

@@ -539,7 +539,7 @@ int bh_sort_particles(void* ctx, const bh_particles* src, const bh_particles* ds
                        (const int32_t*)c->key, N, (const int32_t*)c->offsets, c->cursor, c->idx_a,
                        c->key_sorted, (const double*)src->px, (const double*)src->py, c->pxkey,
                        c->pykey);
-    hipLaunchKernelGGL(k_sort_units, dim3(cdiv(N, SORT_W)), dim3(256), 0, st, F, N,
+    hipLaunchKernelGGL(k_sort_units, dim3(cdiv(N, SORT_W)), dim3(SORT_THREADS), 0, st, F, N,
                        (const int32_t*)c->idx_a, (const int32_t*)c->key_sorted,
                        (const uint64_t*)c->pxkey, (const uint64_t*)c->pykey,
                        (const int32_t*)c->offsets, c->idx_b);

@@ -6,6 +6,7 @@
 #define HIST_BINS 8192     // LDS histogram window (keys) per block
 #define SCAN_TILE 2048     // elements per block in the scans
 #define SORT_W 1024        // within-cell rank window (rows)
+#define SORT_THREADS 1024  // threads of a k_sort_units block (2 rows of the 2 SORT_W window each)
 
 struct SortFields {
     const double *px, *py, *m, *vx, *vy, *ax, *ay;

@@ -502,7 +502,7 @@ __device__ __forceinline__ bool row_less(const SortFields& F, int a, int b)
 // ranks the unit's rows in LDS (all of them lie in [w0, w0 + 2*SORT_W)) on
 // their px keys, then py keys; the remaining fields are read only for rows
 // equal in both.
-__global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_units(SortFields F, int64_t N,
                                                     const int32_t* __restrict__ idx_in,
                                                     const int32_t* __restrict__ key_sorted,
                                                     const uint64_t* __restrict__ pxkey,
@@ -526,14 +526,14 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
     }
     __syncthreads();
     const int64_t wend = min(lend, w0 + 2 * SORT_W);   // > w0: the window holds a row
-    // Each thread's rows t = threadIdx.x + 256 r.  Every load below reads a
+    // Each thread's rows t = threadIdx.x + SORT_THREADS r.  Every load below reads a
     // row clamped into [w0, wend), so it is always in bounds and needs no
     // branch: the loads of all rows are issued before the first is waited on
     // (a guarded load per row would wait for each one in turn).
-    constexpr int RPT = 2 * SORT_W / 256;
+    constexpr int RPT = 2 * SORT_W / SORT_THREADS;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int t = threadIdx.x + 256 * r;
+        const int t = threadIdx.x + SORT_THREADS * r;
         const int64_t q = min(w0 + t, wend - 1);
         // slots past wend get the last row's keys; no rank reads them (every
         // unit this block ranks ends at or before wend)
@@ -546,7 +546,7 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
     int64_t ar[RPT], br[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int64_t q = min(w0 + threadIdx.x + 256 * r, wend - 1);
+        const int64_t q = min(w0 + threadIdx.x + SORT_THREADS * r, wend - 1);
         kr[r] = key_sorted[q];
         er[r] = idx_in[q];
     }
@@ -556,7 +556,7 @@ __global__ __launch_bounds__(256) void k_sort_units(SortFields F, int64_t N,
         br[r] = offsets[kr[r] + 1];
     }
     for (int r = 0; r < RPT; ++r) {
-        const int t = threadIdx.x + 256 * r;
+        const int t = threadIdx.x + SORT_THREADS * r;
         const int64_t q = w0 + t;
         if (q >= wend) break;
         const int64_t a = ar[r], b = br[r];
